@@ -1,0 +1,178 @@
+/*
+ * emst.h -- C-ABI of the MI355X ECL-MST solver (libemst.so).
+ *
+ * The reference (alexfallin/MST-Artifact-SC23, ECL-MST/ECL_MST.cu) has no
+ * FFI: its boundary is the in-file function
+ *     template<bool filter> static bool* gpuMST(const ECLgraph& g, int threshold)
+ * (ECL_MST.cu:265-362) called from main (ECL_MST.cu:432-444) after the host
+ * picks the light-edge threshold (ECL_MST.cu:432-440).  The entry points
+ * below replace that function and its helpers one for one; each cites the
+ * reference code it replaces.  Plain pointers and sizes only.
+ *
+ * Conventions
+ *  - Every entry point returns EMST_OK (0) or a negative EMST_E_* code and
+ *    never exits the process (the reference calls exit(-1) in CheckCuda,
+ *    ECL_MST.cu:254-262, and in readECLgraph, ECLgraph.h:42-59).
+ *  - inMST is one byte per directed CSR edge, 0 or 1, byte-compatible with
+ *    the reference's bool[edges] (ECL_MST.cu:269-270).  Only the
+ *    upper-direction position j (nlist[j] > source) of an MSF edge is 1.
+ *  - A context is bound to one device and one HIP stream and is not
+ *    reentrant; use one context per host thread.
+ */
+#ifndef EMST_H
+#define EMST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EMST_OK 0
+#define EMST_E_ARG (-1)      /* NULL pointer, bad size, bad option            */
+#define EMST_E_GRAPH (-2)    /* CSR fails validation (see emst_validate)      */
+#define EMST_E_HIP (-3)      /* a HIP runtime call failed                      */
+#define EMST_E_NOMEM (-4)    /* device or host allocation failed               */
+#define EMST_E_LIMIT (-5)    /* size beyond a documented limit                 */
+#define EMST_E_STATE (-6)    /* call order (e.g. emst_solve before emst_load)  */
+#define EMST_E_NOWEIGHT (-7) /* graph has no weights and none were synthesized */
+/* .egr reader errors, one per failure branch of ECLgraph.h:42-59 */
+#define EMST_E_OPEN (-10)    /* ECLgraph.h:42 could not open file              */
+#define EMST_E_NODES (-11)   /* ECLgraph.h:43 failed to read nodes             */
+#define EMST_E_EDGES (-12)   /* ECLgraph.h:44 failed to read edges             */
+#define EMST_E_COUNTS (-13)  /* ECLgraph.h:45 node or edge count too low       */
+#define EMST_E_NINDEX (-14)  /* ECLgraph.h:52 failed to read neighbor index    */
+#define EMST_E_NLIST (-15)   /* ECLgraph.h:53 failed to read neighbor list     */
+#define EMST_E_WEIGHTS (-16) /* ECLgraph.h:59 failed to read edge weights      */
+#define EMST_E_WRITE (-17)   /* ECLgraph.h:70-78 write failures                */
+
+/* Read-only view of a graph in the ECLgraph 32-bit CSR layout
+ * (ECLgraph.h:29-35): nindex[nodes+1], nlist[edges], eweight[edges]. */
+typedef struct {
+  int32_t nodes;
+  int32_t edges;
+  const int32_t* nindex;
+  const int32_t* nlist;
+  const int32_t* eweight;
+} emst_graph;
+
+/* Owning graph returned by the reader and the generators. */
+typedef struct {
+  int32_t nodes;
+  int32_t edges;
+  int32_t* nindex;
+  int32_t* nlist;
+  int32_t* eweight; /* NULL when the file had no weight section */
+} emst_graph_buf;
+
+/* Per-solve statistics.  Counts are exact; times come from HIP events on
+ * the context's stream. */
+typedef struct {
+  double ms;                 /* device time, first kernel to last (ECL_MST.cu:301-345) */
+  double init_ms;            /* time inside the worklist-initialisation kernels        */
+  double init_light_ms;      /* of which the light (first) initialisation launch       */
+  double init_heavy_ms;      /* of which the heavy (second) initialisation launch      */
+  int32_t init_launches;     /* number of worklist-initialisation launches timed      */
+  int32_t light_rounds;      /* kernel1-equivalent launches, light phase               */
+  int32_t heavy_rounds;      /* same, heavy phase (0 without the filter)               */
+  int32_t tail_rounds;       /* rounds finished inside the one-workgroup tail kernel   */
+  int64_t light;             /* L: records emitted by the light initialisation         */
+  int64_t heavy;             /* H: heavy records whose components differ               */
+  int64_t mst_edges;         /* edges marked in inMST                                  */
+  uint64_t weight;           /* 64-bit wrapping sum of marked weights (ECL_MST.cu:377) */
+  int64_t round_in_sum;      /* sum over rounds of worklist sizes entering a round     */
+  int64_t round_out_sum;     /* sum over rounds of surviving records                   */
+} emst_stats;
+
+/* Tunables.  Zero in a field selects the built-in default. */
+typedef struct {
+  int32_t block;        /* threads per workgroup for the round kernels (64..1024)  */
+  int32_t grid_per_cu;  /* resident workgroups per CU for grid-stride kernels      */
+  int32_t tile_edges;   /* CSR edges per wave task in the initialisation kernels   */
+  int32_t stage_recs;   /* LDS staging records per wave before a worklist flush    */
+  int32_t tail_max;     /* worklist size at or below which the one-workgroup tail  */
+                        /* kernel finishes the phase (-1 disables the tail)        */
+  int32_t flags;        /* EMST_F_* bits                                           */
+} emst_config;
+
+#define EMST_F_HOST_SYNC 1  /* read the worklist size back every round, as ECL_MST.cu:318 */
+#define EMST_F_NO_GUARD 2   /* unconditional atomicMin (ablation of ECL_MST.cu:222-223) */
+
+typedef struct emst_ctx emst_ctx;
+
+/* Context on HIP device `device` with its own stream.  (ECL_MST.cu:421-427
+ * hard-codes device 0 and the default stream.) */
+int emst_create(int device, emst_ctx** out);
+void emst_destroy(emst_ctx* ctx);
+/* Text of the last error on this context; never NULL. */
+const char* emst_last_error(const emst_ctx* ctx);
+int emst_set_config(emst_ctx* ctx, const emst_config* cfg);
+int emst_get_config(const emst_ctx* ctx, emst_config* cfg);
+
+/* Checks nindex[0]==0, nindex non-decreasing, nindex[nodes]==edges,
+ * 0<=nlist[j]<nodes.  The reference trusts its input; this library does
+ * not, because a bad CSR would make the kernels read out of bounds. */
+int emst_validate(const emst_graph* g);
+
+/* Replaces the threshold selection at ECL_MST.cu:408, 432-443 (hash at
+ * 389-395): *use_filter = (edges/nodes >= 4); *threshold is the
+ * min(19,(int)(3.0*n*s/m))-th smallest of s=min(m,20) hashed weight
+ * samples, or INT32_MAX without the filter. */
+int emst_plain_threshold(const emst_graph* g, int* use_filter, int32_t* threshold);
+
+/* Replaces gpuMST<filter>(g, threshold) (ECL_MST.cu:265-362) as a whole:
+ * copies the CSR to the device, solves, copies inMST[g->edges] back.
+ * use_filter=0 runs gpuMST<false>(g, INT_MAX) regardless of threshold. */
+int emst_gpu_mst(emst_ctx* ctx, const emst_graph* g, int use_filter, int32_t threshold,
+                 uint8_t* inMST, emst_stats* stats);
+
+/* The same call split at the reference's timed region (ECL_MST.cu:301-345):
+ * emst_load does the allocations and host-to-device copies of lines
+ * 268-297 (validating the CSR first), emst_solve is the device-resident
+ * timed region, emst_fetch is the device-to-host copy of line 347.  A
+ * loaded graph stays resident across any number of solves. */
+int emst_load(emst_ctx* ctx, const emst_graph* g);
+int emst_solve(emst_ctx* ctx, int use_filter, int32_t threshold, emst_stats* stats);
+int emst_fetch(emst_ctx* ctx, uint8_t* inMST);
+
+/* Device-pointer form of emst_solve for callers that own device memory
+ * (e.g. tensors).  All pointers are device pointers on the context's
+ * device; d_inMST receives edges bytes.  The CSR is not validated here
+ * (validate on the host first).  stream may be NULL for the context's
+ * own stream. */
+int emst_solve_device(emst_ctx* ctx, int32_t nodes, int32_t edges, const int32_t* d_nindex,
+                      const int32_t* d_nlist, const int32_t* d_eweight, int use_filter,
+                      int32_t threshold, uint8_t* d_inMST, void* stream, emst_stats* stats);
+
+/* Device parent array of the last solve (ints, nodes entries): after a
+ * solve every vertex's root is the smallest vertex id of its tree
+ * (join() at ECL_MST.cu:137-144 hooks the larger root under the smaller). */
+int emst_fetch_parent(emst_ctx* ctx, int32_t* parent);
+
+/* ---- ECLgraph file format: replaces ECLgraph.h:37-90 ------------------- */
+int emst_read_egr(const char* path, emst_graph_buf* out);          /* ECLgraph.h:37-64 */
+int emst_write_egr(const char* path, const emst_graph* g);         /* ECLgraph.h:66-80 */
+void emst_free_graph(emst_graph_buf* g);                           /* ECLgraph.h:82-90 */
+/* Weights for weightless inputs, ECL_MST.cu:410-419: eweight[j] =
+ * |1 + ((i*k) % nodes)| with a wrapping 32-bit multiply.  Allocates
+ * g->eweight (freed by emst_free_graph). */
+int emst_synth_weights(emst_graph_buf* g);
+/* Wrapping 64-bit sum of eweight[j] over inMST[j] != 0 (ECL_MST.cu:377). */
+uint64_t emst_mst_weight(const emst_graph* g, const uint8_t* inMST);
+
+/* Host minimum spanning forest, replacing cpuMST (ECL_MST.cu:82-124):
+ * Kruskal over upper-direction edges in (signed weight, j) order. */
+int emst_cpu_mst(const emst_graph* g, uint8_t* inMST);
+/* Replaces verify (ECL_MST.cu:365-386): returns 0 when the edge sets and
+ * the 64-bit weight sums agree, 1 when they differ, <0 on bad arguments. */
+int emst_verify(const emst_graph* g, const uint8_t* cpu, const uint8_t* gpu, uint64_t* cpu_weight,
+                uint64_t* gpu_weight);
+
+/* Library version string. */
+const char* emst_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EMST_H */

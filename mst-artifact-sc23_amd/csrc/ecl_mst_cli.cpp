@@ -1,0 +1,92 @@
+// ecl_mst_cli.cpp -- command-line driver with the reference's stdout
+// protocol (ECL-MST/ECL_MST.cu:398-455):
+//   input: <path>
+//   9 lines of "%12.9f" seconds, one per device-resident solve
+//   "ERROR: results differ!" (plus a blank line) when the last GPU result
+//   does not match the host Kruskal.
+// Reader failures print the reference's messages (ECLgraph.h:42-59) and
+// exit with -1, as the reference does.  A JSON line with the solver's
+// statistics goes to stderr, leaving stdout in the reference's format.
+
+#include <climits>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/emst.h"
+
+static const char* reader_message(int rc)
+{
+  switch (rc) {
+    case EMST_E_OPEN: return "could not open file";
+    case EMST_E_NODES: return "failed to read nodes";
+    case EMST_E_EDGES: return "failed to read edges";
+    case EMST_E_COUNTS: return "node or edge count too low";
+    case EMST_E_NOMEM: return "memory allocation failed";
+    case EMST_E_NINDEX: return "failed to read neighbor index list";
+    case EMST_E_NLIST: return "failed to read neighbor list";
+    case EMST_E_WEIGHTS: return "failed to read edge weights";
+    default: return "unknown reader error";
+  }
+}
+
+int main(int argc, char* argv[])
+{
+  if (argc != 2) {
+    printf("USAGE: %s input_graph\n", argv[0]);
+    exit(-1);
+  }
+  emst_graph_buf gb;
+  int rc = emst_read_egr(argv[1], &gb);
+  if (rc) {
+    if (rc == EMST_E_OPEN)
+      fprintf(stderr, "ERROR: could not open file %s\n\n", argv[1]);
+    else
+      fprintf(stderr, "ERROR: %s\n\n", reader_message(rc));
+    exit(-1);
+  }
+  printf("input: %s\n", argv[1]);
+  if (!gb.eweight && emst_synth_weights(&gb) != EMST_OK) {
+    fprintf(stderr, "ERROR: memory allocation failed\n\n");
+    exit(-1);
+  }
+  const emst_graph g{gb.nodes, gb.edges, gb.nindex, gb.nlist, gb.eweight};
+  emst_ctx* ctx = nullptr;
+  if (emst_create(0, &ctx) != EMST_OK) {
+    fprintf(stderr, "ERROR: there is no HIP capable device\n\n");
+    exit(-1);
+  }
+  if ((rc = emst_load(ctx, &g)) != EMST_OK) {
+    fprintf(stderr, "ERROR %d: %s\n", rc, emst_last_error(ctx));
+    exit(-1);
+  }
+  const int runs = 9;  // ECL_MST.cu:49
+  emst_stats st{};
+  for (int i = 0; i < runs; i++) {
+    int use_filter = 0;
+    int32_t threshold = INT_MAX;
+    emst_plain_threshold(&g, &use_filter, &threshold);
+    if ((rc = emst_solve(ctx, use_filter, threshold, &st)) != EMST_OK) {
+      fprintf(stderr, "ERROR %d: %s\n", rc, emst_last_error(ctx));
+      exit(-1);
+    }
+    printf("%12.9f\n", st.ms / 1000.0);
+  }
+  std::vector<uint8_t> gpu((size_t)(g.edges ? g.edges : 1)), cpu((size_t)(g.edges ? g.edges : 1));
+  if ((rc = emst_fetch(ctx, gpu.data())) != EMST_OK) {
+    fprintf(stderr, "ERROR %d: %s\n", rc, emst_last_error(ctx));
+    exit(-1);
+  }
+  emst_cpu_mst(&g, cpu.data());
+  uint64_t cw = 0, gw = 0;
+  if (emst_verify(&g, cpu.data(), gpu.data(), &cw, &gw) != 0) printf("ERROR: results differ!\n\n");
+  fprintf(stderr,
+          "{\"nodes\": %d, \"edges\": %d, \"light\": %lld, \"heavy\": %lld, \"light_rounds\": %d, "
+          "\"heavy_rounds\": %d, \"tail_rounds\": %d, \"weight\": %llu, \"last_ms\": %.6f}\n",
+          g.nodes, g.edges, (long long)st.light, (long long)st.heavy, st.light_rounds, st.heavy_rounds,
+          st.tail_rounds, (unsigned long long)gw, st.ms);
+  emst_destroy(ctx);
+  emst_free_graph(&gb);
+  return 0;
+}

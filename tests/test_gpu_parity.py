@@ -1,0 +1,195 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle.
+
+The oracle's serial Kruskal (restating ECL_MST.cu:82-124) fixes the exact
+inMST set for non-negative weights; the oracle's round-schedule emulator
+(restating ECL_MST.cu:127-340) fixes it for any weights and fixes the
+per-phase worklist sizes and round counts."""
+import numpy as np
+import pytest
+
+import emst
+import oracle_lib as O
+from graphs import csr_from_edges, random_graph
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def solver():
+    s = emst.Solver(0)
+    yield s
+    s.close()
+
+
+def G(ni, nl, ew):
+    return emst.Graph(ni, nl, ew)
+
+
+def check(solver, g, use_filter=None, T=None, rounds=True, kruskal=True):
+    if use_filter is None:
+        use_filter, T = emst.plain_threshold(g)
+    print(f"case n={g.nodes} m={g.edges} filter={use_filter} T={T}", flush=True)
+    got, st = solver.gpu_mst(g, use_filter, T)
+    ref, sch = O.schedule(g, use_filter, T)
+    assert np.array_equal(got, ref), "inMST differs from the round-schedule oracle"
+    if kruskal:
+        k = O.kruskal(g)
+        assert np.array_equal(got, k), "inMST differs from Kruskal"
+        rc, cw, gw, oc, og = O.verify(g, k, got)
+        assert rc == 0 and cw == gw
+    assert st.light == sch.light
+    if use_filter:
+        assert st.heavy == sch.heavy
+    if rounds:
+        assert st.light_rounds == sch.light_rounds
+        assert st.heavy_rounds == (sch.heavy_rounds if use_filter else 0)
+    assert st.mst_edges == sch.mst_edges
+    return got, st
+
+
+def test_tiny_fixtures(solver):
+    cases = [
+        (1, []),
+        (2, [(0, 1, 5)]),
+        (3, [(0, 1, 1), (1, 2, 1), (0, 2, 1)]),          # all ties: lowest j wins
+        (4, [(0, 1, 3), (2, 3, 3)]),                      # forest
+        (5, [(0, 1, 2), (1, 2, 2), (2, 3, 2), (3, 4, 2), (0, 4, 1)]),
+        (6, [(0, 1, 7), (0, 1, 3), (1, 2, 3), (2, 0, 3), (4, 5, 0)]),  # multi-edge, zero weight, isolated 3
+    ]
+    for n, edges in cases:
+        g = G(*csr_from_edges(n, edges))
+        for uf, T in ((False, emst.INT_MAX), (True, 2), (True, 0), (True, -1)):
+            check(solver, g, uf, T)
+
+
+def test_random_small(solver):
+    rng = np.random.default_rng(7)
+    for it in range(60):
+        n = int(rng.integers(1, 40))
+        g = G(*random_graph(rng, n, int(rng.integers(0, 120)), 0, int(rng.integers(1, 6))))
+        for uf, T in ((False, emst.INT_MAX), (True, int(rng.integers(0, 6)))):
+            check(solver, g, uf, T)
+
+
+def test_grid_512(solver):
+    g = emst.gen_grid(512, 512, 1, 1, 1 << 20)
+    got, st = check(solver, g)
+    assert got.sum() == 512 * 512 - 1
+
+
+@pytest.mark.parametrize("scale", [12, 16])
+def test_rmat(solver, scale):
+    g = emst.gen_rmat(scale, 16, 0.57, 0.19, 0.19, 2, 1, 1 << 24)
+    check(solver, g)
+    check(solver, g, False, emst.INT_MAX)
+
+
+def test_rmat_heavy_ties(solver):
+    g = emst.gen_rmat(15, 16, 0.57, 0.19, 0.19, 5, 1, 65)
+    check(solver, g)
+
+
+def test_rgg_and_er(solver):
+    check(solver, emst.gen_rgg(1 << 16, 6.0, 3, 1 << 24))
+    check(solver, emst.gen_er(20000, 10.0, 4, 1, 1 << 20))
+
+
+def test_negative_weights_follow_gpu_order(solver):
+    # The reference GPU orders keys as unsigned (ECL_MST.cu:221), so a
+    # negative weight sorts above every non-negative one there; the
+    # schedule oracle models that, Kruskal (signed) does not.
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        n = int(rng.integers(2, 30))
+        g = G(*random_graph(rng, n, 60, -5, 6))
+        for uf, T in ((False, emst.INT_MAX), (True, 0), (True, -3)):
+            check(solver, g, uf, T, kruskal=False)
+
+
+def test_asymmetric_input(solver):
+    rng = np.random.default_rng(11)
+    for _ in range(20):
+        n = int(rng.integers(2, 50))
+        g = G(*random_graph(rng, n, 150, 1, 9, symmetric=False))
+        check(solver, g, False, emst.INT_MAX)
+        check(solver, g, True, 4)
+
+
+def test_modes_agree(solver):
+    g = emst.gen_rmat(14, 16, 0.57, 0.19, 0.19, 9, 1, 1 << 10)
+    uf, T = emst.plain_threshold(g)
+    base, _ = solver.gpu_mst(g, uf, T)
+    for kw in (dict(flags=emst.F_HOST_SYNC), dict(flags=emst.F_NO_GUARD), dict(tail_max=-1),
+               dict(tail_max=1 << 20), dict(stage_recs=64), dict(grid_per_cu=1)):
+        print("mode", kw, flush=True)
+        solver.set_config(**kw)
+        try:
+            got, _ = solver.gpu_mst(g, uf, T)
+        finally:
+            solver.set_config()
+        assert np.array_equal(got, base), kw
+
+
+def test_repeat_and_roots(solver):
+    g = emst.gen_rmat(14, 8, 0.57, 0.19, 0.19, 12, 1, 100)
+    uf, T = emst.plain_threshold(g)
+    solver.load(g)
+    outs = []
+    for _ in range(3):
+        solver.solve(uf, T)
+        outs.append(solver.fetch())
+    assert all(np.array_equal(outs[0], o) for o in outs)
+    parent = solver.fetch_parent(g.nodes)
+    # root of every vertex is the smallest id of its tree
+    roots = parent.copy()
+    for _ in range(64):
+        roots = roots[roots]
+    src = g.sources()
+    mst = outs[0].astype(bool)
+    comp = np.arange(g.nodes)
+    # union-find over MST edges on the host
+    p = np.arange(g.nodes)
+
+    def f(x):
+        while p[x] != x:
+            p[x] = p[p[x]]
+            x = p[x]
+        return x
+    for u, v in zip(src[mst], g.nlist[mst]):
+        a, b = f(u), f(v)
+        if a != b:
+            p[max(a, b)] = min(a, b)
+    comp = np.array([f(x) for x in range(g.nodes)])
+    assert np.array_equal(roots, comp)
+
+
+def test_solve_device_pointers(solver):
+    """emst_solve_device on caller-owned device memory.  The buffers come
+    from the same HIP runtime libemst.so links (/opt/rocm), through ctypes;
+    torch ships its own HIP runtime and cannot share the process."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    H2D, D2H = 1, 2
+    g = emst.gen_grid(64, 80, 3, 1, 100)
+    bufs = []
+
+    def dev(arr):
+        p = C.c_void_p()
+        assert hip.hipMalloc(C.byref(p), max(arr.nbytes, 1)) == 0
+        bufs.append(p)
+        assert hip.hipMemcpy(p, arr.ctypes.data, arr.nbytes, H2D) == 0
+        return p.value
+    try:
+        out = np.full(g.edges, 7, np.uint8)
+        d_out = dev(out)
+        solver.solve_device(g.nodes, g.edges, dev(g.nindex), dev(g.nlist), dev(g.eweight), False, emst.INT_MAX,
+                            d_out)
+        got = np.zeros(g.edges, np.uint8)
+        assert hip.hipMemcpy(got.ctypes.data, C.c_void_p(d_out), g.edges, D2H) == 0
+    finally:
+        for p in bufs:
+            hip.hipFree(p)
+    assert np.array_equal(got, O.kruskal(g))
