@@ -98,6 +98,7 @@ typedef struct {
 
 #define EMST_F_HOST_SYNC 1  /* read the worklist size back every round, as ECL_MST.cu:318 */
 #define EMST_F_NO_GUARD 2   /* unconditional atomicMin (ablation of ECL_MST.cu:222-223) */
+#define EMST_F_NO_COMBINE 4 /* no per-workgroup LDS min-combining of keys (ablation)     */
 
 typedef struct emst_ctx emst_ctx;
 

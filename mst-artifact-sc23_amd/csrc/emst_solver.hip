@@ -180,41 +180,127 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
 // k_select (kernel1, ECL_MST.cu:210-226): for each record find both roots;
 // records inside one component are dropped, the rest are written back with
 // their roots (implicit path compression) and offer their key to both roots.
+//
+// Two MI355X-specific changes, both invisible in the results:
+//  * Survivors of a 2048-record tile are compacted with a workgroup scan and
+//    reserved with one atomicAdd per tile (a per-record or per-wave counter
+//    atomic serialises on one address).
+//  * Keys are first min-combined per workgroup in an LDS table keyed by
+//    root (4-probe open addressing, claimed slots are never released).  A hub
+//    component that appears in most records is claimed early and then
+//    costs one global atomicMin per workgroup instead of one per record;
+//    device-scope atomics and coherent loads are resolved beyond the
+//    per-XCD L2, so a hot address otherwise serialises the whole grid.
+//    Roots that find no slot fall back to the guarded global atomic.
 // ---------------------------------------------------------------------------
+constexpr int kSelItems = 4;
+constexpr int kSelTile = kRoundWaves * kWave * kSelItems;
+constexpr int kTbl = 1024;
+
+__device__ __forceinline__ void offer_key(int* tags, u64* keys, u64* minv, int rep, u64 key, bool guard,
+                                          bool combine)
+{
+  if (combine) {
+    const unsigned h = ((unsigned)rep * 0x9E3779B1u) >> (32 - 10);
+#pragma unroll
+    for (int pr = 0; pr < 4; pr++) {
+      const unsigned sl = (h + pr) & (kTbl - 1);
+      int t = tags[sl];
+      if (t == -1) {
+        t = atomicCAS(&tags[sl], -1, rep);
+        if (t == -1) t = rep;
+      }
+      if (t == rep) {
+        atomicMin(&keys[sl], key);
+        return;
+      }
+    }
+  }
+  min_key(minv, rep, key, guard);
+}
+
 __global__ __launch_bounds__(kRoundWaves * kWave) void k_select(
     const int4* __restrict__ wl_in, const int* __restrict__ cnt_in, int4* __restrict__ wl_out,
-    int* __restrict__ cnt_out, const int* __restrict__ parent, u64* __restrict__ minv, int guard,
+    int* __restrict__ cnt_out, const int* __restrict__ parent, u64* __restrict__ minv, int guard, int combine,
     long long* __restrict__ stats, int rounds_slot)
 {
-  __shared__ int4 lds_stage[kRoundWaves * 64];
+  __shared__ int s_tags[kTbl];
+  __shared__ u64 s_keys[kTbl];
+  __shared__ int s_wtot[kRoundWaves];
+  __shared__ int s_woff[kRoundWaves];
+  __shared__ int s_base;
   const int n_in = *cnt_in;
   if (n_in == 0) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  const int tid = threadIdx.x;
+  if (blockIdx.x == 0 && tid == 0) {
     stats[rounds_slot] += 1;
     stats[ST_IN_SUM] += n_in;
   }
-  const int lane = lane_id();
-  const int wid = threadIdx.x / kWave;
-  Stage st{lds_stage + wid * 64, 64, 0};
-  const long long stride = (long long)gridDim.x * kRoundWaves * kWave;
-  for (long long b = ((long long)blockIdx.x * kRoundWaves + wid) * kWave; b < n_in; b += stride) {
-    const long long i = b + lane;
-    const bool valid = i < n_in;
-    int4 r = valid ? wl_in[i] : make_int4(0, 0, 0, 0);
-    int a = 0, c = 0;
-    if (valid) {
-      a = find_root(r.x, parent);
-      c = find_root(r.y, parent);
-    }
-    const bool keep = valid && a != c;
-    if (keep) {
-      const u64 key = make_key(r.z, r.w);
-      min_key(minv, a, key, guard);
-      min_key(minv, c, key, guard);
-    }
-    stage_push(st, keep, make_int4(a, c, r.z, r.w), wl_out, cnt_out);
+  for (int i = tid; i < kTbl; i += blockDim.x) {
+    s_tags[i] = -1;
+    s_keys[i] = kNoKey;
   }
-  stage_flush(st, wl_out, cnt_out);
+  __syncthreads();
+  const int lane = lane_id();
+  const int wid = tid / kWave;
+  const long long ntiles = ((long long)n_in + kSelTile - 1) / kSelTile;
+  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const long long base = t * kSelTile;
+    int4 r[kSelItems];
+#pragma unroll
+    for (int k = 0; k < kSelItems; k++) {
+      const long long i = base + k * (kRoundWaves * kWave) + tid;
+      r[k] = i < n_in ? wl_in[i] : make_int4(0, 0, 0, 0);
+    }
+    bool keep[kSelItems];
+    int mine = 0;
+#pragma unroll
+    for (int k = 0; k < kSelItems; k++) {
+      const long long i = base + k * (kRoundWaves * kWave) + tid;
+      keep[k] = false;
+      if (i < n_in) {
+        const int a = find_root(r[k].x, parent);
+        const int c = find_root(r[k].y, parent);
+        if (a != c) {
+          keep[k] = true;
+          r[k].x = a;
+          r[k].y = c;
+          const u64 key = make_key(r[k].z, r[k].w);
+          offer_key(s_tags, s_keys, minv, a, key, guard, combine);
+          offer_key(s_tags, s_keys, minv, c, key, guard, combine);
+        }
+      }
+      mine += keep[k];
+    }
+    // workgroup exclusive scan of the per-thread survivor counts
+    int x = mine;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    if (lane == kWave - 1) s_wtot[wid] = x;
+    __syncthreads();
+    if (tid == 0) {
+      int acc = 0;
+      for (int w = 0; w < kRoundWaves; w++) {
+        s_woff[w] = acc;
+        acc += s_wtot[w];
+      }
+      s_base = acc ? atomicAdd(cnt_out, acc) : 0;
+    }
+    __syncthreads();
+    long long pos = (long long)s_base + s_woff[wid] + (x - mine);
+#pragma unroll
+    for (int k = 0; k < kSelItems; k++)
+      if (keep[k]) wl_out[pos++] = r[k];
+    __syncthreads();
+  }
+  // publish the workgroup's combined minima
+  for (int i = tid; i < kTbl; i += blockDim.x) {
+    const int rep = s_tags[i];
+    if (rep >= 0) min_key(minv, rep, s_keys[i], guard);
+  }
 }
 
 // One workgroup finishes a phase: join the current survivors, then repeat
@@ -484,6 +570,7 @@ static int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_n
   const int T = use_filter ? threshold : INT_MAX;
   const bool host_sync = (cfg.flags & EMST_F_HOST_SYNC) != 0;
   const int guard = (cfg.flags & EMST_F_NO_GUARD) ? 0 : 1;
+  const int combine = (cfg.flags & EMST_F_NO_COMBINE) ? 0 : 1;
   const int tail_max = cfg.tail_max < 0 ? 0 : cfg.tail_max;
   const int round_grid = std::max(1, ctx->cus * std::max(1, cfg.grid_per_cu));
   const int init_grid = std::max(1, ctx->cus * 8);
@@ -533,9 +620,9 @@ static int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_n
       HIPCHK(hipStreamSynchronize(s));
       while (size > 0) {
         HIPCHK(hipMemsetAsync(&d_cnt[cur ^ 1], 0, 4, s));
-        const int blocks = (int)std::min<long long>(((long long)size + 511) / 512, round_grid);
+        const int blocks = (int)std::min<long long>(((long long)size + kSelTile - 1) / kSelTile, round_grid);
         hipLaunchKernelGGL(k_select, dim3(blocks), dim3(kRoundWaves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[cur ^ 1],
-                           &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, d_stats, rounds_slot);
+                           &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot);
         cur ^= 1;
         HIPCHK(hipMemcpyAsync(&size, &d_cnt[cur], 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -556,7 +643,7 @@ static int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_n
       const int in = r & 1, out = in ^ 1;
       ctx->h_flag[slot] = -1;
       hipLaunchKernelGGL(k_select, dim3(round_grid), dim3(kRoundWaves * kWave), 0, s, wl[in], &d_cnt[in], wl[out],
-                         &d_cnt[out], ctx->d_parent, ctx->d_minv, guard, d_stats, rounds_slot);
+                         &d_cnt[out], ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot);
       hipLaunchKernelGGL(k_join, dim3(round_grid), dim3(kRoundWaves * kWave), 0, s, wl[out], &d_cnt[out], wl[in],
                          &d_cnt[in], ctx->d_parent, ctx->d_minv, d_inMST, tail_max, ctx->d_flag + slot, d_stats,
                          rounds_slot);

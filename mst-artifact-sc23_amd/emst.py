@@ -25,6 +25,7 @@ ERRORS = {
 }
 F_HOST_SYNC = 1
 F_NO_GUARD = 2
+F_NO_COMBINE = 4
 INT_MAX = 2**31 - 1
 
 

@@ -119,7 +119,8 @@ def test_modes_agree(solver):
     g = emst.gen_rmat(14, 16, 0.57, 0.19, 0.19, 9, 1, 1 << 10)
     uf, T = emst.plain_threshold(g)
     base, _ = solver.gpu_mst(g, uf, T)
-    for kw in (dict(flags=emst.F_HOST_SYNC), dict(flags=emst.F_NO_GUARD), dict(tail_max=-1),
+    for kw in (dict(flags=emst.F_HOST_SYNC), dict(flags=emst.F_NO_GUARD), dict(flags=emst.F_NO_COMBINE),
+               dict(flags=emst.F_NO_GUARD | emst.F_NO_COMBINE), dict(tail_max=-1),
                dict(tail_max=1 << 20), dict(stage_recs=64), dict(grid_per_cu=1)):
         print("mode", kw, flush=True)
         solver.set_config(**kw)
