@@ -94,18 +94,27 @@ __global__ __launch_bounds__(256) void k_zero(uint8_t* __restrict__ p, long long
 // ---------------------------------------------------------------------------
 // k_init_wl: streams the CSR and emits worklist records.
 //   HEAVY=false: {v, u, w, j} for u > v and w <= T      (ECL_MST.cu:170-176)
-//   HEAVY=true : {root(v), root(u), w, j} for u > v, w > T and different
-//                roots (ECL_MST.cu:166, 173-174); parent is flat here.
+//   HEAVY=true : {root(v), root(u), w, j} for u > v with different roots
+//                (ECL_MST.cu:166, 173-174); parent is flat here.  The
+//                reference also tests w > T (ECL_MST.cu:197); after the
+//                light phase every light edge lies inside one component, so
+//                "roots differ" already implies w > T and the weight is
+//                read only for crossing edges.  bigmap marks the members of
+//                one large component (2 MB for 2^24 vertices, resident in
+//                L2); an edge with both ends marked is skipped without
+//                touching the root array.
 // One wave task covers kGroupEdges consecutive CSR positions.  Lane i takes
 // positions base+i, base+64+i, ... so every load is one coalesced 256-byte
-// request.  The source vertex of a position is found against a window of
-// 64 vertex boundaries held one per lane: src(p) = max{v : nindex[v] <= p}.
+// request, and all kSub loads of a task are issued before the first use.
+// The source vertex of a position is found against a window of 64 vertex
+// boundaries held one per lane: src(p) = max{v : nindex[v] <= p}.
 // ---------------------------------------------------------------------------
 template <bool HEAVY>
 __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
     const int* __restrict__ nindex, const int* __restrict__ nlist, const int* __restrict__ eweight,
     const int* __restrict__ gsrc, int nodes, int edges, int T, const int* __restrict__ label,
-    int4* __restrict__ wl, int* __restrict__ cnt, int stage_cap, long long* __restrict__ stats)
+    const unsigned* __restrict__ bigmap, int4* __restrict__ wl, int* __restrict__ cnt, int stage_cap,
+    long long* __restrict__ stats)
 {
   extern __shared__ int4 lds_stage[];
   const int lane = lane_id();
@@ -116,13 +125,14 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
   long long emitted = 0;
   for (long long g = (long long)blockIdx.x * kInitWaves + wid; g < ngroups; g += nwaves) {
     const long long gbeg = g * kGroupEdges;
-    int nl[kSub];
+    int nl[kSub], w[kSub];
+    int base = gsrc[g];
 #pragma unroll
     for (int u = 0; u < kSub; u++) {
       const long long p = gbeg + u * kWave + lane;
       nl[u] = p < edges ? nlist[p] : -1;
+      if (!HEAVY) w[u] = p < edges ? eweight[p] : 0;
     }
-    int base = gsrc[g];
     long long bi = (long long)base + 1 + lane;
     int bnd = bi <= nodes ? nindex[bi] : INT_MAX;
     int src[kSub];
@@ -146,34 +156,65 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
       }
       src[u] = s;
     }
-    int w[kSub];
+    if (!HEAVY) {
 #pragma unroll
-    for (int u = 0; u < kSub; u++) {
-      const long long p = gbeg + u * kWave + lane;
-      w[u] = (nl[u] > src[u]) ? eweight[p] : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < kSub; u++) {
-      const int j = (int)(gbeg + u * kWave + lane);
-      const bool up = nl[u] > src[u];
-      bool keep;
-      int4 rec;
-      if (!HEAVY) {
-        keep = up && w[u] <= T;
-        rec = make_int4(src[u], nl[u], w[u], j);
-      } else {
-        const bool hv = up && w[u] > T;
-        const int a = hv ? label[src[u]] : 0;
-        const int b = hv ? label[nl[u]] : 0;
-        keep = hv && a != b;
-        rec = make_int4(a, b, w[u], j);
+      for (int u = 0; u < kSub; u++) {
+        const int j = (int)(gbeg + u * kWave + lane);
+        const bool keep = nl[u] > src[u] && w[u] <= T;
+        emitted += keep;
+        stage_push(st, keep, make_int4(src[u], nl[u], w[u], j), wl, cnt);
       }
-      emitted += keep;
-      stage_push(st, keep, rec, wl, cnt);
+    } else {
+      bool both[kSub];
+#pragma unroll
+      for (int u = 0; u < kSub; u++) {
+        both[u] = true;
+        if (nl[u] > src[u]) {
+          const unsigned bu = bigmap[(unsigned)nl[u] >> 5] >> (nl[u] & 31);
+          const unsigned bv = bigmap[(unsigned)src[u] >> 5] >> (src[u] & 31);
+          both[u] = (bu & bv & 1u) != 0;
+        }
+      }
+      int a[kSub], b[kSub];
+#pragma unroll
+      for (int u = 0; u < kSub; u++) {
+        a[u] = 0;
+        b[u] = 0;
+        if (!both[u]) {
+          a[u] = label[src[u]];
+          b[u] = label[nl[u]];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kSub; u++) {
+        const long long p = gbeg + u * kWave + lane;
+        const bool keep = !both[u] && a[u] != b[u];
+        const int wt = keep ? eweight[p] : 0;
+        emitted += keep;
+        stage_push(st, keep, make_int4(a[u], b[u], wt, (int)p), wl, cnt);
+      }
     }
   }
   stage_flush(st, wl, cnt);
   if (emitted) atomicAdd((unsigned long long*)&stats[HEAVY ? ST_HEAVY : ST_LIGHT], (unsigned long long)emitted);
+}
+
+// Picks the component that the heavy pass' bitmap marks: the most frequent
+// root among 64 evenly spaced vertices (one wave).  Any choice is correct;
+// a frequent one saves the most root lookups.
+__global__ __launch_bounds__(64) void k_pick_big(int nodes, const int* __restrict__ parent, int* __restrict__ out)
+{
+  const int lane = lane_id();
+  const int v = (int)(((long long)lane * nodes) / kWave + ((long long)nodes / (2 * kWave)));
+  const int r = find_root(v < nodes ? v : nodes - 1, parent);
+  int c = 0;
+  for (int k = 0; k < kWave; k++) c += __shfl(r, k) == r;
+  int best = (c << 8) | (kWave - 1 - lane);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) best = max(best, __shfl_xor(best, off));
+  const int winner = kWave - 1 - (best & 0xff);
+  const int big = __shfl(r, winner);
+  if (lane == 0) out[0] = big;
 }
 
 // ---------------------------------------------------------------------------
@@ -411,18 +452,28 @@ __global__ __launch_bounds__(kRoundWaves * kWave) void k_join(
   }
 }
 
-// parent[v] = root(v).  Concurrent writers only shorten chains toward the
-// same root, so readers stay correct.
-__global__ __launch_bounds__(256) void k_flatten(int nodes, int* __restrict__ parent, int* __restrict__ cnt)
+// parent[v] = root(v), and bit v of bigmap = (root(v) == *big).  Concurrent
+// writers only shorten chains toward the same root, so readers stay correct.
+__global__ __launch_bounds__(256) void k_flatten(int nodes, int* __restrict__ parent, int* __restrict__ cnt,
+                                                 const int* __restrict__ big, unsigned* __restrict__ bigmap)
 {
   const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (v == 0) {
     cnt[0] = 0;
     cnt[1] = 0;
   }
+  const int bigroot = *big;
+  int r = -1;
   if (v < nodes) {
-    const int r = find_root((int)v, parent);
+    r = find_root((int)v, parent);
     parent[v] = r;
+  }
+  const u64 m = __ballot(r == bigroot && r >= 0);
+  const long long v0 = v - lane_id();
+  const long long words = ((long long)nodes + 31) / 32;
+  if (lane_id() == 0 && v0 < nodes) {
+    bigmap[v0 >> 5] = (unsigned)m;
+    if ((v0 >> 5) + 1 < words) bigmap[(v0 >> 5) + 1] = (unsigned)(m >> 32);
   }
 }
 
@@ -470,6 +521,7 @@ struct emst_ctx {
   // work buffers, grown on demand and reused
   int* d_parent = nullptr;
   u64* d_minv = nullptr;
+  unsigned* d_bigmap = nullptr;
   size_t cap_nodes = 0;
   int4* d_wl[2] = {nullptr, nullptr};
   size_t cap_wl = 0;
@@ -524,11 +576,14 @@ static int ensure_buffers(emst_ctx* ctx, int32_t nodes, int32_t edges, long long
   if ((size_t)nodes > ctx->cap_nodes || !ctx->d_parent || !ctx->d_minv) {
     if (ctx->d_parent) (void)hipFree(ctx->d_parent);
     if (ctx->d_minv) (void)hipFree(ctx->d_minv);
+    if (ctx->d_bigmap) (void)hipFree(ctx->d_bigmap);
     ctx->d_parent = nullptr;
     ctx->d_minv = nullptr;
+    ctx->d_bigmap = nullptr;
     ctx->cap_nodes = 0;
     if (hipMalloc((void**)&ctx->d_parent, (size_t)std::max(nodes, 1) * 4) != hipSuccess ||
-        hipMalloc((void**)&ctx->d_minv, (size_t)std::max(nodes, 1) * 8) != hipSuccess)
+        hipMalloc((void**)&ctx->d_minv, (size_t)std::max(nodes, 1) * 8) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_bigmap, ((size_t)std::max(nodes, 1) + 63) / 32 * 4) != hipSuccess)
       return fail(ctx, EMST_E_NOMEM, "hipMalloc of parent/minv failed");
     ctx->cap_nodes = (size_t)std::max(nodes, 1);
   }
@@ -597,16 +652,18 @@ static int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_n
   for (int phase = 0; phase < (use_filter ? 2 : 1); phase++) {
     if (phase == 1) {
       const int blocks = (int)(((long long)nodes + 255) / 256);
-      hipLaunchKernelGGL(k_flatten, dim3(blocks), dim3(256), 0, s, nodes, ctx->d_parent, d_cnt);
+      hipLaunchKernelGGL(k_pick_big, dim3(1), dim3(kWave), 0, s, nodes, ctx->d_parent, &d_cnt[2]);
+      hipLaunchKernelGGL(k_flatten, dim3(blocks), dim3(256), 0, s, nodes, ctx->d_parent, d_cnt, &d_cnt[2],
+                         ctx->d_bigmap);
     }
     if (edges > 0) {
       HIPCHK(hipEventRecord(ctx->ev[2 + 2 * phase], s));
       if (phase == 0)
         hipLaunchKernelGGL(k_init_wl<false>, dim3(init_grid), dim3(kInitWaves * kWave), init_lds, s, d_nindex, d_nlist,
-                           d_eweight, ctx->d_gsrc, nodes, edges, T, ctx->d_parent, wl[0], &d_cnt[0], stage_cap, d_stats);
+                           d_eweight, ctx->d_gsrc, nodes, edges, T, ctx->d_parent, ctx->d_bigmap, wl[0], &d_cnt[0], stage_cap, d_stats);
       else
         hipLaunchKernelGGL(k_init_wl<true>, dim3(init_grid), dim3(kInitWaves * kWave), init_lds, s, d_nindex, d_nlist,
-                           d_eweight, ctx->d_gsrc, nodes, edges, T, ctx->d_parent, wl[0], &d_cnt[0], stage_cap, d_stats);
+                           d_eweight, ctx->d_gsrc, nodes, edges, T, ctx->d_parent, ctx->d_bigmap, wl[0], &d_cnt[0], stage_cap, d_stats);
       HIPCHK(hipEventRecord(ctx->ev[3 + 2 * phase], s));
       HIPCHK(hipGetLastError());
       init_launches++;
@@ -739,7 +796,7 @@ void emst_destroy(emst_ctx* ctx)
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   void* ptrs[] = {ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, ctx->d_inMST, ctx->d_parent, ctx->d_minv,
-                  ctx->d_wl[0],  ctx->d_wl[1], ctx->d_gsrc,    ctx->d_cnt,   ctx->d_stats};
+                  ctx->d_wl[0],  ctx->d_wl[1], ctx->d_gsrc,    ctx->d_cnt,   ctx->d_stats, ctx->d_bigmap};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
