@@ -260,6 +260,7 @@ __device__ __forceinline__ void offer_key(int* tags, u64* keys, u64* minv, int r
   min_key(minv, rep, key, guard);
 }
 
+template <bool FIRST>
 __global__ __launch_bounds__(kRoundWaves * kWave) void k_select(
     const int4* __restrict__ wl_in, const int* __restrict__ cnt_in, int4* __restrict__ wl_out,
     int* __restrict__ cnt_out, const int* __restrict__ parent, u64* __restrict__ minv, int guard, int combine,
@@ -300,8 +301,11 @@ __global__ __launch_bounds__(kRoundWaves * kWave) void k_select(
       const long long i = base + k * (kRoundWaves * kWave) + tid;
       keep[k] = false;
       if (i < n_in) {
-        const int a = find_root(r[k].x, parent);
-        const int c = find_root(r[k].y, parent);
+        // In a phase's first round every endpoint is a root and every
+        // record crosses (initialisation guarantees both), so the finds
+        // and the copy are skipped: the list is joined in place.
+        const int a = FIRST ? r[k].x : find_root(r[k].x, parent);
+        const int c = FIRST ? r[k].y : find_root(r[k].y, parent);
         if (a != c) {
           keep[k] = true;
           r[k].x = a;
@@ -313,6 +317,7 @@ __global__ __launch_bounds__(kRoundWaves * kWave) void k_select(
       }
       mine += keep[k];
     }
+    if (FIRST) continue;
     // workgroup exclusive scan of the per-thread survivor counts
     int x = mine;
 #pragma unroll
@@ -337,7 +342,9 @@ __global__ __launch_bounds__(kRoundWaves * kWave) void k_select(
       if (keep[k]) wl_out[pos++] = r[k];
     __syncthreads();
   }
-  // publish the workgroup's combined minima
+  // publish the workgroup's combined minima (after every offer of every
+  // wave: the first-round pass has no per-tile barrier)
+  __syncthreads();
   for (int i = tid; i < kTbl; i += blockDim.x) {
     const int rep = s_tags[i];
     if (rep >= 0) min_key(minv, rep, s_keys[i], guard);
@@ -675,18 +682,23 @@ static int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_n
       int size = 0;
       HIPCHK(hipMemcpyAsync(&size, &d_cnt[cur], 4, hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
+      if (size > 0) {
+        const int blocks = (int)std::min<long long>(((long long)size + kSelTile - 1) / kSelTile, round_grid);
+        hipLaunchKernelGGL(k_select<true>, dim3(blocks), dim3(kRoundWaves * kWave), 0, s, wl[cur], &d_cnt[cur],
+                           wl[cur ^ 1], &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, combine, d_stats,
+                           rounds_slot);
+      }
       while (size > 0) {
+        hipLaunchKernelGGL(k_join_plain, dim3((size + 255) / 256), dim3(256), 0, s, wl[cur], size, ctx->d_parent,
+                           ctx->d_minv, d_inMST);
         HIPCHK(hipMemsetAsync(&d_cnt[cur ^ 1], 0, 4, s));
         const int blocks = (int)std::min<long long>(((long long)size + kSelTile - 1) / kSelTile, round_grid);
-        hipLaunchKernelGGL(k_select, dim3(blocks), dim3(kRoundWaves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[cur ^ 1],
-                           &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot);
+        hipLaunchKernelGGL(k_select<false>, dim3(blocks), dim3(kRoundWaves * kWave), 0, s, wl[cur], &d_cnt[cur],
+                           wl[cur ^ 1], &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, combine, d_stats,
+                           rounds_slot);
         cur ^= 1;
         HIPCHK(hipMemcpyAsync(&size, &d_cnt[cur], 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        if (size > 0) {
-          hipLaunchKernelGGL(k_join_plain, dim3((size + 255) / 256), dim3(256), 0, s, wl[cur], size, ctx->d_parent,
-                             ctx->d_minv, d_inMST);
-        }
       }
       HIPCHK(hipMemsetAsync(d_cnt, 0, 8, s));
       continue;
@@ -697,13 +709,18 @@ static int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_n
     int first_slot = slot;
     for (;; r++) {
       if (slot >= kHflagSlots) return fail(ctx, EMST_E_LIMIT, "round limit exceeded");
-      const int in = r & 1, out = in ^ 1;
+      // list `cur` holds the survivors of the last select; the phase's
+      // first select is the offer-only pass over the initial list
+      const int cur = r & 1, nxt = cur ^ 1;
       ctx->h_flag[slot] = -1;
-      hipLaunchKernelGGL(k_select, dim3(round_grid), dim3(kRoundWaves * kWave), 0, s, wl[in], &d_cnt[in], wl[out],
-                         &d_cnt[out], ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot);
-      hipLaunchKernelGGL(k_join, dim3(round_grid), dim3(kRoundWaves * kWave), 0, s, wl[out], &d_cnt[out], wl[in],
-                         &d_cnt[in], ctx->d_parent, ctx->d_minv, d_inMST, tail_max, ctx->d_flag + slot, d_stats,
+      if (r == 0)
+        hipLaunchKernelGGL(k_select<true>, dim3(round_grid), dim3(kRoundWaves * kWave), 0, s, wl[0], &d_cnt[0], wl[1],
+                           &d_cnt[1], ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot);
+      hipLaunchKernelGGL(k_join, dim3(round_grid), dim3(kRoundWaves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt],
+                         &d_cnt[nxt], ctx->d_parent, ctx->d_minv, d_inMST, tail_max, ctx->d_flag + slot, d_stats,
                          rounds_slot);
+      hipLaunchKernelGGL(k_select<false>, dim3(round_grid), dim3(kRoundWaves * kWave), 0, s, wl[cur], &d_cnt[cur],
+                         wl[nxt], &d_cnt[nxt], ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot);
       HIPCHK(hipGetLastError());
       slot++;
       if (r >= lag) {
