@@ -1,0 +1,222 @@
+"""Host-side tests of libemst.so (no GPU): exported C-ABI, .egr reader and
+writer, weight synthesis, plain threshold, host Kruskal, validation, the
+workload generators, and the CLI's pre-device error paths."""
+import hashlib
+import json
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import emst
+import oracle_lib as O
+from graphs import csr_from_edges, random_graph
+from test_oracle import egr_fixtures
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CLI = os.path.join(ROOT, "mst-artifact-sc23_amd", "bin", "ecl_mst")
+GOLD = json.load(open(os.path.join(HERE, "golden", "golden.json")))
+GEN_HASHES = json.load(open(os.path.join(HERE, "golden", "gen_hashes.json")))
+
+
+def declared_functions():
+    names = []
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^\s*[A-Za-z_][\w\s\*]*?\b(emst_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 20
+    out = subprocess.run(["nm", "-D", "--defined-only", emst.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (emst_\w+)", out))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    for n in names:
+        assert hasattr(emst.lib, n)
+
+
+ORC_TO_EMST = {0: 0, -1: -10, -2: -11, -3: -12, -4: -13, -6: -14, -7: -15, -8: -16}
+
+
+def test_reader_matches_oracle_on_fixtures(tmp_path):
+    for name, path, rc in egr_fixtures(str(tmp_path)):
+        if rc != 0:
+            with pytest.raises(emst.EmstError) as e:
+                emst.read_egr(path)
+            assert e.value.code == ORC_TO_EMST[rc], name
+            continue
+        g = emst.read_egr(path)
+        _, ni, nl, ew = O.read_egr(path)
+        assert np.array_equal(g.nindex, ni) and np.array_equal(g.nlist, nl), name
+        assert (g.eweight is None) == (ew is None), name
+        if ew is not None:
+            assert np.array_equal(g.eweight, ew), name
+
+
+def test_write_read_roundtrip(tmp_path):
+    g = emst.gen_rmat(10, 8, seed=3)
+    p = str(tmp_path / "g.egr")
+    emst.write_egr(p, g)
+    assert os.path.getsize(p) == 8 + 4 * (g.nodes + 1) + 8 * g.edges
+    h = emst.read_egr(p)
+    assert np.array_equal(h.nindex, g.nindex) and np.array_equal(h.nlist, g.nlist)
+    assert np.array_equal(h.eweight, g.eweight)
+    # weightless write: the reader then reports no weights
+    emst.write_egr(p, emst.Graph(g.nindex, g.nlist, None))
+    assert emst.read_egr(p).eweight is None
+    with pytest.raises(emst.EmstError):
+        emst.write_egr(str(tmp_path / "nodir" / "x.egr"), g)
+
+
+def test_synth_weights_match_oracle_and_golden(tmp_path):
+    g = emst.gen_rmat(11, 8, seed=4)
+    s = emst.synth_weights(g)
+    assert np.array_equal(s.eweight, O.synth_weights(g.nindex, g.nlist))
+    for i, k, n, w in GOLD["synth"]:
+        nindex = np.zeros(n + 1, np.int32)
+        nindex[i + 1:] = 1
+        got = emst.synth_weights(emst.Graph(nindex, np.array([k], np.int32)))
+        assert int(got.eweight[0]) == w
+    # read_egr(synth_weights=True) on a weightless file
+    p = str(tmp_path / "nw.egr")
+    emst.write_egr(p, emst.Graph(g.nindex, g.nlist, None))
+    assert np.array_equal(emst.read_egr(p, synth_weights=True).eweight, s.eweight)
+
+
+def test_plain_threshold_matches_oracle():
+    rng = np.random.default_rng(9)
+    for _ in range(200):
+        n = int(rng.integers(1, 60))
+        g = emst.Graph(*random_graph(rng, n, int(rng.integers(0, 400)), -3, int(rng.integers(1, 1000))))
+        assert emst.plain_threshold(g) == O.plain_threshold(g.nodes, g.eweight)
+    for scale in (10, 14):
+        g = emst.gen_rmat(scale, 16, seed=2)
+        assert emst.plain_threshold(g) == O.plain_threshold(g.nodes, g.eweight)
+
+
+def test_cpu_mst_matches_oracle():
+    rng = np.random.default_rng(10)
+    for _ in range(100):
+        n = int(rng.integers(1, 200))
+        g = emst.Graph(*random_graph(rng, n, int(rng.integers(0, 600)), -2, int(rng.integers(1, 20))))
+        assert np.array_equal(emst.cpu_mst(g), O.kruskal(g))
+    for g in (emst.gen_grid(64, 64, 1), emst.gen_rmat(14, 16, seed=2), emst.gen_rgg(20000, 6.0, 3)):
+        k = O.kruskal(g)
+        c = emst.cpu_mst(g)
+        assert np.array_equal(c, k)
+        differs, cw, gw = emst.verify(g, k, c)
+        assert not differs and cw == gw == emst.mst_weight(g, c)
+
+
+def test_verify_reports_difference():
+    g = emst.gen_grid(8, 8, 2)
+    k = emst.cpu_mst(g)
+    bad = k.copy()
+    bad[np.nonzero(k)[0][0]] = 0
+    differs, cw, gw = emst.verify(g, k, bad)
+    assert differs and cw > gw
+
+
+def test_validate_rejects_bad_csr():
+    ni, nl, ew = csr_from_edges(4, [(0, 1, 1), (2, 3, 1)])
+    assert emst.validate(emst.Graph(ni, nl, ew)) == 0
+    bad = nl.copy(); bad[0] = 4
+    assert emst.validate(emst.Graph(ni, bad, ew)) == -2
+    bad = nl.copy(); bad[0] = -1
+    assert emst.validate(emst.Graph(ni, bad, ew)) == -2
+    badi = ni.copy(); badi[2] = badi[3] + 1
+    assert emst.validate(emst.Graph(badi, nl, ew)) == -2
+    badi = ni.copy(); badi[-1] -= 1          # nindex[nodes] != edges
+    assert emst.validate(emst.Graph(badi, nl, ew)) == -2
+    badi = ni.copy(); badi[0] = 1            # nindex[0] != 0
+    assert emst.validate(emst.Graph(badi, nl, ew)) == -2
+
+
+def _digest(g):
+    h = hashlib.sha256()
+    for a in (g.nindex, g.nlist, g.eweight):
+        h.update(a.tobytes())
+    return h.hexdigest()
+
+
+def _check_structure(g, wlo, whi):
+    src = g.sources().astype(np.int64)
+    dst = g.nlist.astype(np.int64)
+    assert np.all(src != dst), "self-loop"
+    # sorted, duplicate-free lists
+    key = src * g.nodes + dst
+    assert np.all(np.diff(key) > 0)
+    # symmetric with mirrored weights
+    rev = dst * g.nodes + src
+    order = np.argsort(rev)
+    assert np.array_equal(rev[order], key)
+    assert np.array_equal(g.eweight[order], g.eweight)
+    assert g.eweight.min() >= wlo and g.eweight.max() < whi
+
+
+def test_generators_structure_and_golden_digests():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mk", os.path.join(HERE, "golden", "make_gen_hashes.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    for name, f in mk.CASES.items():
+        g = f()
+        assert _digest(g) == GEN_HASHES[name], name
+    _check_structure(emst.gen_grid(16, 24, 1, 1, 1 << 20), 1, 1 << 20)
+    _check_structure(emst.gen_rmat(12, 16, 0.57, 0.19, 0.19, 2, 1, 1 << 24), 1, 1 << 24)
+    _check_structure(emst.gen_rmat(10, 16, 0.57, 0.19, 0.19, 5, 1, 65), 1, 65)
+    _check_structure(emst.gen_rgg(5000, 6.0, 3, 1 << 24), 1, 1 << 24)
+    _check_structure(emst.gen_er(3000, 10.0, 4, 1, 1 << 20), 1, 1 << 20)
+
+
+def test_generators_independent_of_thread_count():
+    try:
+        emst.set_gen_threads(1)
+        a = [_digest(emst.gen_rmat(11, 16, seed=7)), _digest(emst.gen_rgg(6000, 6.0, 7))]
+        emst.set_gen_threads(5)
+        b = [_digest(emst.gen_rmat(11, 16, seed=7)), _digest(emst.gen_rgg(6000, 6.0, 7))]
+    finally:
+        emst.set_gen_threads(0)
+    assert a == b
+
+
+def test_grid_size_matches_reference_input_count():
+    # 2d-2e20.sym has 4,190,208 directed edges (generate_compare_tables.py:16)
+    g = emst.gen_grid(1024, 1024, 1)
+    assert g.edges == GOLD["grid_2d_2e20_edges"] == 4 * 1024 * 1023
+    assert emst.gen_grid(512, 512, 1).edges == 1046528
+
+
+def test_rgg_mean_degree_and_rmat_size():
+    g = emst.gen_rgg(1 << 16, 6.0, 3)
+    assert 5.5 < g.edges / g.nodes < 6.1   # boundary effects lower it slightly
+    g = emst.gen_rmat(12, 16, seed=2)
+    assert g.nodes == 4096 and g.edges <= 2 * 16 * 4096
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="CLI not built")
+def test_cli_error_paths(tmp_path):
+    r = subprocess.run([CLI], capture_output=True, text=True)
+    assert r.returncode == 255 and r.stdout.startswith("USAGE:")
+    r = subprocess.run([CLI, str(tmp_path / "missing.egr")], capture_output=True, text=True)
+    assert r.returncode == 255 and "could not open file" in r.stderr
+    p = tmp_path / "bad.egr"
+    p.write_bytes(np.array([0, 0], np.int32).tobytes())
+    r = subprocess.run([CLI, str(p)], capture_output=True, text=True)
+    assert r.returncode == 255 and "node or edge count too low" in r.stderr
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True)
+    assert r.returncode == 2 and "one GPU" in r.stderr
