@@ -83,6 +83,10 @@ typedef struct {
   uint64_t weight;           /* 64-bit wrapping sum of marked weights (ECL_MST.cu:377) */
   int64_t round_in_sum;      /* sum over rounds of worklist sizes entering a round     */
   int64_t round_out_sum;     /* sum over rounds of surviving records                   */
+  int32_t threshold_used;    /* light-edge threshold the solve used (INT32_MAX = none)  */
+  int32_t filter_used;       /* 1 when the two-phase light/heavy path ran              */
+  float learned_q;           /* quantile predicted by the network, -1 if not used      */
+  int32_t reserved;
 } emst_stats;
 
 /* Tunables.  Zero in a field selects the built-in default. */
@@ -99,6 +103,12 @@ typedef struct {
 #define EMST_F_HOST_SYNC 1  /* read the worklist size back every round, as ECL_MST.cu:318 */
 #define EMST_F_NO_GUARD 2   /* unconditional atomicMin (ablation of ECL_MST.cu:222-223) */
 #define EMST_F_NO_COMBINE 4 /* no per-workgroup LDS min-combining of keys (ablation)     */
+
+/* Filter modes for the use_filter argument of the solve entry points. */
+#define EMST_FILTER_OFF 0     /* gpuMST<false>(g, INT_MAX)                                  */
+#define EMST_FILTER_PLAIN 1   /* gpuMST<true>(g, threshold), threshold from the caller       */
+#define EMST_FILTER_LEARNED 2 /* threshold predicted on the device by the loaded network;    */
+                              /* without a network, the plain rule of ECL_MST.cu:432-443     */
 
 typedef struct emst_ctx emst_ctx;
 
@@ -124,7 +134,8 @@ int emst_plain_threshold(const emst_graph* g, int* use_filter, int32_t* threshol
 
 /* Replaces gpuMST<filter>(g, threshold) (ECL_MST.cu:265-362) as a whole:
  * copies the CSR to the device, solves, copies inMST[g->edges] back.
- * use_filter=0 runs gpuMST<false>(g, INT_MAX) regardless of threshold. */
+ * use_filter is an EMST_FILTER_* mode; EMST_FILTER_OFF runs
+ * gpuMST<false>(g, INT_MAX) regardless of threshold. */
 int emst_gpu_mst(emst_ctx* ctx, const emst_graph* g, int use_filter, int32_t threshold,
                  uint8_t* inMST, emst_stats* stats);
 
@@ -150,6 +161,33 @@ int emst_solve_device(emst_ctx* ctx, int32_t nodes, int32_t edges, const int32_t
  * solve every vertex's root is the smallest vertex id of its tree
  * (join() at ECL_MST.cu:137-144 hooks the larger root under the smaller). */
 int emst_fetch_parent(emst_ctx* ctx, int32_t* parent);
+
+/* ---- learned light-edge threshold (no counterpart in the reference) -----
+ * A 32-64-64-1 network predicts log2 of the light-edge quantile from 32
+ * features of the resident graph; the threshold is that quantile of 1024
+ * hashed weight samples.  Any threshold gives the same exact MSF; the
+ * prediction only changes the work.  Parameters are EMST_MLP_PARAMS fp32
+ * values (W1[32][64], b1[64], W2[64][64], b2[64], W3[64], b3), used as
+ * bf16 with fp32 accumulation. */
+#define EMST_MLP_PARAMS 6337
+#define EMST_MLP_FEATURES 32
+#define EMST_WEIGHT_SAMPLES 1024
+int emst_set_filter_model(emst_ctx* ctx, const float* params); /* NULL removes the model */
+/* Runs feature extraction and inference on the loaded graph.  Any output
+ * pointer may be NULL; features has EMST_MLP_FEATURES floats, samples
+ * EMST_WEIGHT_SAMPLES sorted weights. */
+int emst_learned_threshold(emst_ctx* ctx, int* use_filter, int32_t* threshold, float* q, float* features,
+                           int32_t* samples);
+/* Training kernels (device `device`): forward of a batch, gradient of the
+ * mean squared error of a batch, and full-batch Adam training for `epochs`
+ * steps from params_inout (fp32 master weights, bf16 working copy). */
+int emst_mlp_forward_batch(int device, int32_t batch, const float* params, const float* X, float* Y);
+int emst_mlp_grad_batch(int device, int32_t batch, const float* params, const float* X, const float* T,
+                        float* grad, float* loss);
+int emst_mlp_train(int device, int32_t batch, const float* X, const float* T, int32_t epochs, float lr,
+                   uint64_t seed, float* params_inout, float* final_loss);
+/* bf16 bits (round to nearest even) of an fp32 value. */
+uint16_t emst_bf16_bits(float x);
 
 /* ---- ECLgraph file format: replaces ECLgraph.h:37-90 ------------------- */
 int emst_read_egr(const char* path, emst_graph_buf* out);          /* ECLgraph.h:37-64 */

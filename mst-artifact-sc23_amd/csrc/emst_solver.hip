@@ -30,12 +30,14 @@
 #include <algorithm>
 #include <climits>
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/emst.h"
 #include "emst_kernels.hpp"
+#include "learned.hpp"
 #include "host_util.hpp"
 
 using namespace emst;
@@ -114,9 +116,14 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
     const int* __restrict__ nindex, const int* __restrict__ nlist, const int* __restrict__ eweight,
     const int* __restrict__ gsrc, int nodes, int edges, int T, const int* __restrict__ label,
     const unsigned* __restrict__ bigmap, int4* __restrict__ wl, int* __restrict__ cnt, int stage_cap,
-    long long* __restrict__ stats)
+    long long* __restrict__ stats, const int* __restrict__ dthr)
 {
   extern __shared__ int4 lds_stage[];
+  // dthr: threshold and filter flag decided on the device (learned mode)
+  if (dthr) {
+    if (HEAVY && dthr[1] == 0) return;
+    T = dthr[0];
+  }
   const int lane = lane_id();
   const int wid = threadIdx.x / kWave;
   Stage st{lds_stage + wid * stage_cap, stage_cap, 0};
@@ -462,13 +469,15 @@ __global__ __launch_bounds__(kRoundWaves * kWave) void k_join(
 // parent[v] = root(v), and bit v of bigmap = (root(v) == *big).  Concurrent
 // writers only shorten chains toward the same root, so readers stay correct.
 __global__ __launch_bounds__(256) void k_flatten(int nodes, int* __restrict__ parent, int* __restrict__ cnt,
-                                                 const int* __restrict__ big, unsigned* __restrict__ bigmap)
+                                                 const int* __restrict__ big, unsigned* __restrict__ bigmap,
+                                                 const int* __restrict__ dthr)
 {
   const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (v == 0) {
     cnt[0] = 0;
     cnt[1] = 0;
   }
+  if (dthr && dthr[1] == 0) return;  // unfiltered: no heavy phase
   const int bigroot = *big;
   int r = -1;
   if (v < nodes) {
@@ -540,6 +549,13 @@ struct emst_ctx {
   int* d_flag = nullptr;
   hipEvent_t ev[8] = {};
   int32_t last_nodes = 0;
+  // learned threshold
+  unsigned short* d_mlp16 = nullptr;  // bf16 parameters
+  int have_model = 0;
+  int* d_thr = nullptr;               // [threshold, use_filter] decided on the device
+  float* d_q = nullptr;
+  float* d_feat = nullptr;
+  int* d_samples = nullptr;
 };
 
 namespace {
@@ -629,7 +645,10 @@ static int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_n
   int rc = ensure_buffers(ctx, nodes, edges, wl_cap);
   if (rc) return rc;
   const emst_config& cfg = ctx->cfg;
-  const int T = use_filter ? threshold : INT_MAX;
+  const bool learned = use_filter == EMST_FILTER_LEARNED;
+  if (use_filter < 0 || use_filter > EMST_FILTER_LEARNED) return fail(ctx, EMST_E_ARG, "bad filter mode");
+  const int T = use_filter == EMST_FILTER_PLAIN ? threshold : INT_MAX;
+  const int* dthr = learned ? ctx->d_thr : nullptr;
   const bool host_sync = (cfg.flags & EMST_F_HOST_SYNC) != 0;
   const int guard = (cfg.flags & EMST_F_NO_GUARD) ? 0 : 1;
   const int combine = (cfg.flags & EMST_F_NO_COMBINE) ? 0 : 1;
@@ -645,6 +664,9 @@ static int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_n
   ctx->last_nodes = nodes;
 
   HIPCHK(hipEventRecord(ctx->ev[0], s));
+  if (learned)
+    launch_learned_threshold(s, nodes, edges, d_nindex, d_eweight, ctx->d_mlp16, ctx->have_model, ctx->d_thr,
+                             ctx->d_q, ctx->d_feat, ctx->d_samples);
   {
     const int blocks = (int)(((long long)nodes + 255) / 256);
     hipLaunchKernelGGL(k_setup, dim3(blocks), dim3(256), 0, s, nodes, d_nindex, ctx->d_parent, ctx->d_minv,
@@ -657,20 +679,23 @@ static int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_n
   int init_launches = 0;
   int slot = 0;
   for (int phase = 0; phase < (use_filter ? 2 : 1); phase++) {
+    if (host_sync && learned) return fail(ctx, EMST_E_ARG, "learned filter needs the device-driven loop");
     if (phase == 1) {
       const int blocks = (int)(((long long)nodes + 255) / 256);
       hipLaunchKernelGGL(k_pick_big, dim3(1), dim3(kWave), 0, s, nodes, ctx->d_parent, &d_cnt[2]);
       hipLaunchKernelGGL(k_flatten, dim3(blocks), dim3(256), 0, s, nodes, ctx->d_parent, d_cnt, &d_cnt[2],
-                         ctx->d_bigmap);
+                         ctx->d_bigmap, dthr);
     }
     if (edges > 0) {
       HIPCHK(hipEventRecord(ctx->ev[2 + 2 * phase], s));
       if (phase == 0)
         hipLaunchKernelGGL(k_init_wl<false>, dim3(init_grid), dim3(kInitWaves * kWave), init_lds, s, d_nindex, d_nlist,
-                           d_eweight, ctx->d_gsrc, nodes, edges, T, ctx->d_parent, ctx->d_bigmap, wl[0], &d_cnt[0], stage_cap, d_stats);
+                           d_eweight, ctx->d_gsrc, nodes, edges, T, ctx->d_parent, ctx->d_bigmap, wl[0], &d_cnt[0], stage_cap, d_stats,
+                           dthr);
       else
         hipLaunchKernelGGL(k_init_wl<true>, dim3(init_grid), dim3(kInitWaves * kWave), init_lds, s, d_nindex, d_nlist,
-                           d_eweight, ctx->d_gsrc, nodes, edges, T, ctx->d_parent, ctx->d_bigmap, wl[0], &d_cnt[0], stage_cap, d_stats);
+                           d_eweight, ctx->d_gsrc, nodes, edges, T, ctx->d_parent, ctx->d_bigmap, wl[0], &d_cnt[0], stage_cap, d_stats,
+                           dthr);
       HIPCHK(hipEventRecord(ctx->ev[3 + 2 * phase], s));
       HIPCHK(hipGetLastError());
       init_launches++;
@@ -752,6 +777,19 @@ static int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_n
     long long hs[ST_COUNT];
     HIPCHK(hipMemcpy(hs, d_stats, sizeof(hs), hipMemcpyDeviceToHost));
     std::memset(stats, 0, sizeof(*stats));
+    if (learned) {
+      int thr[2];
+      float q;
+      HIPCHK(hipMemcpy(thr, ctx->d_thr, 8, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(&q, ctx->d_q, 4, hipMemcpyDeviceToHost));
+      stats->threshold_used = thr[0];
+      stats->filter_used = thr[1];
+      stats->learned_q = q;
+    } else {
+      stats->threshold_used = T;
+      stats->filter_used = use_filter ? 1 : 0;
+      stats->learned_q = -1.f;
+    }
     stats->ms = t;
     stats->init_ms = init_ms;
     stats->init_light_ms = tph[0];
@@ -793,6 +831,10 @@ int emst_create(int device, emst_ctx** out)
   ok = ok && hipHostMalloc((void**)&ctx->h_flag, kHflagSlots * sizeof(int), hipHostMallocMapped) == hipSuccess;
   ok = ok && hipHostGetDevicePointer((void**)&ctx->d_flag, ctx->h_flag, 0) == hipSuccess;
   for (int i = 0; ok && i < 8; i++) ok = hipEventCreate(&ctx->ev[i]) == hipSuccess;
+  ok = ok && hipMalloc((void**)&ctx->d_mlp16, kParams * 2) == hipSuccess &&
+       hipMalloc((void**)&ctx->d_thr, 8) == hipSuccess && hipMalloc((void**)&ctx->d_q, 4) == hipSuccess &&
+       hipMalloc((void**)&ctx->d_feat, kIn * 4) == hipSuccess &&
+       hipMalloc((void**)&ctx->d_samples, kWeightSamples * 4) == hipSuccess;
   if (ok) {
     ok = hipFuncSetAttribute((const void*)k_init_wl<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)ctx->lds_per_block) == hipSuccess &&
@@ -813,7 +855,8 @@ void emst_destroy(emst_ctx* ctx)
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   void* ptrs[] = {ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, ctx->d_inMST, ctx->d_parent, ctx->d_minv,
-                  ctx->d_wl[0],  ctx->d_wl[1], ctx->d_gsrc,    ctx->d_cnt,   ctx->d_stats, ctx->d_bigmap};
+                  ctx->d_wl[0],  ctx->d_wl[1], ctx->d_gsrc,    ctx->d_cnt,   ctx->d_stats, ctx->d_bigmap, ctx->d_mlp16, ctx->d_thr, ctx->d_q, ctx->d_feat,
+                  ctx->d_samples};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
@@ -952,6 +995,46 @@ int emst_gpu_mst(emst_ctx* ctx, const emst_graph* g, int use_filter, int32_t thr
     for (int32_t j = 0; j < g->edges; j++) stats->mst_edges += inMST[j] != 0;
     stats->weight = emst_mst_weight(g, inMST);
   }
+  return EMST_OK;
+}
+
+int emst_set_filter_model(emst_ctx* ctx, const float* params)
+{
+  if (!ctx) return EMST_E_ARG;
+  if (!params) {
+    ctx->have_model = 0;
+    return EMST_OK;
+  }
+  std::vector<uint16_t> h(kParams);
+  for (int i = 0; i < kParams; i++) {
+    if (!std::isfinite(params[i])) return fail(ctx, EMST_E_ARG, "non-finite model parameter");
+    h[i] = emst_bf16_bits(params[i]);
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipMemcpy(ctx->d_mlp16, h.data(), kParams * 2, hipMemcpyHostToDevice));
+  ctx->have_model = 1;
+  return EMST_OK;
+}
+
+int emst_learned_threshold(emst_ctx* ctx, int* use_filter, int32_t* threshold, float* q, float* features,
+                           int32_t* samples)
+{
+  if (!ctx) return EMST_E_ARG;
+  if (!ctx->loaded) return fail(ctx, EMST_E_STATE, "emst_learned_threshold before emst_load");
+  HIPCHK(hipSetDevice(ctx->device));
+  launch_learned_threshold(ctx->stream, ctx->n, ctx->m, ctx->d_nindex, ctx->d_eweight, ctx->d_mlp16, ctx->have_model,
+                           ctx->d_thr, ctx->d_q, ctx->d_feat, ctx->d_samples);
+  HIPCHK(hipGetLastError());
+  int thr[2];
+  float qq;
+  HIPCHK(hipMemcpyAsync(thr, ctx->d_thr, 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(&qq, ctx->d_q, 4, hipMemcpyDeviceToHost, ctx->stream));
+  if (features) HIPCHK(hipMemcpyAsync(features, ctx->d_feat, kIn * 4, hipMemcpyDeviceToHost, ctx->stream));
+  if (samples) HIPCHK(hipMemcpyAsync(samples, ctx->d_samples, kWeightSamples * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (use_filter) *use_filter = thr[1];
+  if (threshold) *threshold = thr[0];
+  if (q) *q = qq;
   return EMST_OK;
 }
 

@@ -1,0 +1,448 @@
+// learned.hip -- learned light-edge threshold (no counterpart in the
+// reference, which picks the threshold with a 20-sample heuristic,
+// ECL_MST.cu:432-440).
+//
+// Features (32 floats, computed on the device from the resident CSR):
+//   0  log2(n) / 32          1  log2(m + 1) / 32
+//   2  log2(m / n + 1) / 8   3  fraction of sampled vertices with degree 0
+//   4..15  degree histogram of kVertexSamples hashed vertices, bins
+//          {1, 2-3, 4-7, ..., 2048+} as fractions (bin k: 2^k <= deg < 2^(k+1))
+//   16..31 sorted weight samples at ranks 32(2i+1) of kWeightSamples,
+//          divided by the largest sample
+// The network is 32-64-64-1 with ReLU, bf16 weights and fp32
+// accumulation.  Its output y is log2 of the light-edge quantile q; the
+// threshold is the sorted weight sample of rank floor(q * 1023), and
+// q >= kNoFilterQ selects the unfiltered path (threshold INT_MAX).
+// The MST stays exact for any threshold; only the work changes.
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/emst.h"
+#include "learned.hpp"
+
+namespace emst {
+
+__device__ __forceinline__ unsigned ecl_hash(unsigned v)  // ECL_MST.cu:390-395
+{
+  v = ((v >> 16) ^ v) * 0x45d9f3bu;
+  v = ((v >> 16) ^ v) * 0x45d9f3bu;
+  return (v >> 16) ^ v;
+}
+
+// bf16 round-to-nearest-even of an fp32 value, and back.
+__device__ __forceinline__ float bf16_round(float x)
+{
+  unsigned u = __float_as_uint(x);
+  if ((u & 0x7f800000u) == 0x7f800000u) return x;  // inf / nan unchanged
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return __uint_as_float(u & 0xffff0000u);
+}
+
+// One workgroup of kFeatThreads threads: samples, sorts, builds features.
+__device__ void features_block(int nodes, int edges, const int* __restrict__ nindex,
+                               const int* __restrict__ eweight, float* feat, int* samples_sorted)
+{
+  __shared__ int s_w[kWeightSamples];
+  __shared__ int s_hist[kDegBins + 1];
+  const int t = threadIdx.x;
+  if (t <= kDegBins) s_hist[t] = 0;
+  for (int i = t; i < kWeightSamples; i += blockDim.x)
+    s_w[i] = edges > 0 ? eweight[ecl_hash((unsigned)i) % (unsigned)edges] : 0;
+  __syncthreads();
+  for (int i = t; i < kVertexSamples; i += blockDim.x) {
+    const unsigned v = ecl_hash((unsigned)i ^ 0x5bd1e995u) % (unsigned)nodes;
+    const int deg = nindex[v + 1] - nindex[v];
+    int bin = 0;  // 0 = isolated
+    if (deg > 0) {
+      bin = 1 + (31 - __clz(deg));
+      if (bin > kDegBins) bin = kDegBins;
+    }
+    atomicAdd(&s_hist[bin], 1);
+  }
+  // bitonic sort of the weight samples (kWeightSamples is a power of two)
+  for (int k = 2; k <= kWeightSamples; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      __syncthreads();
+      for (int i = t; i < kWeightSamples; i += blockDim.x) {
+        const int l = i ^ j;
+        if (l > i) {
+          const bool up = (i & k) == 0;
+          const int a = s_w[i], b = s_w[l];
+          if ((a > b) == up) {
+            s_w[i] = b;
+            s_w[l] = a;
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = t; i < kWeightSamples; i += blockDim.x) samples_sorted[i] = s_w[i];
+  if (t == 0) {
+    const double n = nodes, m = edges;
+    feat[0] = (float)(log2(n) / 32.0);
+    feat[1] = (float)(log2(m + 1.0) / 32.0);
+    feat[2] = (float)(log2(m / n + 1.0) / 8.0);
+    feat[3] = (float)s_hist[0] / kVertexSamples;
+    for (int b = 1; b <= kDegBins; b++) feat[3 + b] = (float)s_hist[b] / kVertexSamples;
+    const float wmax = s_w[kWeightSamples - 1] > 0 ? (float)s_w[kWeightSamples - 1] : 1.0f;
+    for (int i = 0; i < 16; i++) {
+      const float v = (float)s_w[32 * (2 * i + 1)] / wmax;
+      feat[16 + i] = v > 0.f ? v : 0.f;
+    }
+  }
+  __syncthreads();
+}
+
+// Forward pass of one sample; p holds bf16-valued fp32 parameters.
+__device__ float mlp_forward_one(const float* __restrict__ p, const float* x, float* h1, float* h2)
+{
+  for (int i = 0; i < kH; i++) {
+    float a = p[kB1 + i];
+    for (int f = 0; f < kIn; f++) a = fmaf(x[f], p[kW1 + f * kH + i], a);
+    h1[i] = a > 0.f ? a : 0.f;
+  }
+  for (int k = 0; k < kH; k++) {
+    float a = p[kB2 + k];
+    for (int i = 0; i < kH; i++) a = fmaf(h1[i], p[kW2 + i * kH + k], a);
+    h2[k] = a > 0.f ? a : 0.f;
+  }
+  float y = p[kB3];
+  for (int k = 0; k < kH; k++) y = fmaf(h2[k], p[kW3 + k], y);
+  return y;
+}
+
+// Plain threshold on the device (ECL_MST.cu:408, 432-443), the fallback
+// when no model is loaded: 20 hashed samples, insertion-sorted.
+__device__ void plain_threshold_dev(int nodes, int edges, const int* __restrict__ eweight, int* T, int* use_filter)
+{
+  if (edges / nodes >= 4) {
+    int sw[20];
+    const int ns = edges < 20 ? edges : 20;
+    for (int i = 0; i < ns; i++) {
+      const int v = eweight[ecl_hash((unsigned)i) % (unsigned)edges];
+      int k = i;
+      while (k > 0 && sw[k - 1] > v) {
+        sw[k] = sw[k - 1];
+        k--;
+      }
+      sw[k] = v;
+    }
+    int idx = (int)(3.0 * nodes * ns / edges);
+    if (idx > 19) idx = 19;
+    *T = sw[idx];
+    *use_filter = 1;
+  } else {
+    *T = INT_MAX;
+    *use_filter = 0;
+  }
+}
+
+// Features, inference and threshold for the resident graph; writes
+// out[0] = threshold, out[1] = use_filter, outf[0] = q (or -1 when the
+// plain fallback decided), feat[32], samples[1024].  Parameters are read
+// as bf16 (p16) and widened to fp32; each hidden unit is one thread.
+__global__ __launch_bounds__(kFeatThreads) void k_learned_threshold(
+    int nodes, int edges, const int* __restrict__ nindex, const int* __restrict__ eweight,
+    const unsigned short* __restrict__ p16, int have_model, int* __restrict__ out, float* __restrict__ outf,
+    float* __restrict__ feat, int* __restrict__ samples)
+{
+  __shared__ float s_p[kParams];
+  __shared__ float s_feat[kIn];
+  __shared__ float s_h1[kH], s_h2[kH];
+  const int t = threadIdx.x;
+  for (int i = t; i < kParams; i += blockDim.x) s_p[i] = have_model ? __uint_as_float((unsigned)p16[i] << 16) : 0.f;
+  features_block(nodes, edges, nindex, eweight, s_feat, samples);
+  if (t < kIn) feat[t] = s_feat[t];
+  if (t < kH) {
+    float a = s_p[kB1 + t];
+    for (int f = 0; f < kIn; f++) a = fmaf(s_feat[f], s_p[kW1 + f * kH + t], a);
+    s_h1[t] = a > 0.f ? a : 0.f;
+  }
+  __syncthreads();
+  if (t < kH) {
+    float a = s_p[kB2 + t];
+    for (int i = 0; i < kH; i++) a = fmaf(s_h1[i], s_p[kW2 + i * kH + t], a);
+    s_h2[t] = a > 0.f ? a : 0.f;
+  }
+  __syncthreads();
+  if (t == 0) {
+    int use_filter = 0, T = INT_MAX;
+    float q = -1.f;
+    if (!have_model) {
+      plain_threshold_dev(nodes, edges, eweight, &T, &use_filter);
+    } else if (edges > 0) {
+      float y = s_p[kB3];
+      for (int k = 0; k < kH; k++) y = fmaf(s_h2[k], s_p[kW3 + k], y);
+      if (!(y == y)) y = 0.f;  // NaN: unfiltered (exact either way)
+      y = fminf(fmaxf(y, kMinLog2Q), 0.f);
+      q = exp2f(y);
+      if (q < kNoFilterQ) {
+        int r = (int)floorf(q * (kWeightSamples - 1));
+        r = r < 0 ? 0 : (r >= kWeightSamples ? kWeightSamples - 1 : r);
+        use_filter = 1;
+        T = samples[r];
+      }
+    }
+    out[0] = T;
+    out[1] = use_filter;
+    outf[0] = q;
+  }
+}
+
+// ---- training ---------------------------------------------------------
+// Forward for a batch: one thread per sample; stores activations.
+__global__ __launch_bounds__(256) void k_mlp_fwd(int B, const float* __restrict__ p, const float* __restrict__ X,
+                                                 float* __restrict__ H1, float* __restrict__ H2,
+                                                 float* __restrict__ Y)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B) return;
+  float x[kIn], h1[kH], h2[kH];
+  for (int f = 0; f < kIn; f++) x[f] = X[(long long)s * kIn + f];
+  const float y = mlp_forward_one(p, x, h1, h2);
+  for (int i = 0; i < kH; i++) {
+    H1[(long long)s * kH + i] = h1[i];
+    H2[(long long)s * kH + i] = h2[i];
+  }
+  Y[s] = y;
+}
+
+// Per-sample deltas of the mean squared error L = mean((y - t)^2).
+__global__ __launch_bounds__(256) void k_mlp_delta(int B, const float* __restrict__ p, const float* __restrict__ Y,
+                                                   const float* __restrict__ Tg, const float* __restrict__ H1,
+                                                   const float* __restrict__ H2, float* __restrict__ D1,
+                                                   float* __restrict__ D2, float* __restrict__ DY,
+                                                   float* __restrict__ sq)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B) return;
+  const float e = Y[s] - Tg[s];
+  sq[s] = e * e;
+  const float dy = 2.f * e / (float)B;
+  DY[s] = dy;
+  float d2[kH];
+  for (int k = 0; k < kH; k++) {
+    const float g = dy * p[kW3 + k];
+    d2[k] = H2[(long long)s * kH + k] > 0.f ? g : 0.f;
+    D2[(long long)s * kH + k] = d2[k];
+  }
+  for (int i = 0; i < kH; i++) {
+    float g = 0.f;
+    for (int k = 0; k < kH; k++) g = fmaf(p[kW2 + i * kH + k], d2[k], g);
+    D1[(long long)s * kH + i] = H1[(long long)s * kH + i] > 0.f ? g : 0.f;
+  }
+}
+
+// Gradient of every parameter: one thread per parameter, summing over the
+// batch in sample order (deterministic).
+__global__ __launch_bounds__(256) void k_mlp_grad(int B, const float* __restrict__ X, const float* __restrict__ H1,
+                                                  const float* __restrict__ H2, const float* __restrict__ D1,
+                                                  const float* __restrict__ D2, const float* __restrict__ DY,
+                                                  float* __restrict__ G)
+{
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= kParams) return;
+  float g = 0.f;
+  if (q < kB1) {  // W1[f][i]
+    const int f = q / kH, i = q % kH;
+    for (int s = 0; s < B; s++) g = fmaf(X[(long long)s * kIn + f], D1[(long long)s * kH + i], g);
+  } else if (q < kW2) {
+    const int i = q - kB1;
+    for (int s = 0; s < B; s++) g += D1[(long long)s * kH + i];
+  } else if (q < kB2) {  // W2[i][k]
+    const int i = (q - kW2) / kH, k = (q - kW2) % kH;
+    for (int s = 0; s < B; s++) g = fmaf(H1[(long long)s * kH + i], D2[(long long)s * kH + k], g);
+  } else if (q < kW3) {
+    const int k = q - kB2;
+    for (int s = 0; s < B; s++) g += D2[(long long)s * kH + k];
+  } else if (q < kB3) {
+    const int k = q - kW3;
+    for (int s = 0; s < B; s++) g = fmaf(H2[(long long)s * kH + k], DY[s], g);
+  } else {
+    for (int s = 0; s < B; s++) g += DY[s];
+  }
+  G[q] = g;
+}
+
+// Adam on fp32 master weights; the working copy used by the forward pass
+// is the bf16 rounding of the master weights.
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ master, float* __restrict__ work,
+                                              const float* __restrict__ G, float* __restrict__ M,
+                                              float* __restrict__ V, float lr, float b1, float b2, float eps,
+                                              float c1, float c2)
+{
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= kParams) return;
+  const float g = G[q];
+  const float m = b1 * M[q] + (1.f - b1) * g;
+  const float v = b2 * V[q] + (1.f - b2) * g * g;
+  M[q] = m;
+  V[q] = v;
+  const float w = master[q] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+  master[q] = w;
+  work[q] = bf16_round(w);
+}
+
+__global__ void k_bf16_copy(int n, const float* __restrict__ src, float* __restrict__ dst)
+{
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) dst[q] = bf16_round(src[q]);
+}
+
+}  // namespace emst
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+using namespace emst;
+
+namespace {
+
+struct TrainBufs {
+  float *p = nullptr, *work = nullptr, *G = nullptr, *M = nullptr, *V = nullptr;
+  float *X = nullptr, *T = nullptr, *Y = nullptr, *H1 = nullptr, *H2 = nullptr, *D1 = nullptr, *D2 = nullptr,
+        *DY = nullptr, *sq = nullptr;
+  ~TrainBufs()
+  {
+    for (float* q : {p, work, G, M, V, X, T, Y, H1, H2, D1, D2, DY, sq})
+      if (q) (void)hipFree(q);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+uint16_t emst_bf16_bits(float x)
+{
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+int emst_mlp_forward_batch(int device, int32_t batch, const float* params, const float* X, float* Y)
+{
+  if (batch < 1 || !params || !X || !Y) return EMST_E_ARG;
+  if (hipSetDevice(device) != hipSuccess) return EMST_E_HIP;
+  TrainBufs b;
+  bool ok = hipMalloc((void**)&b.p, kParams * 4) == hipSuccess && hipMalloc((void**)&b.work, kParams * 4) == hipSuccess &&
+            hipMalloc((void**)&b.X, (size_t)batch * kIn * 4) == hipSuccess &&
+            hipMalloc((void**)&b.Y, (size_t)batch * 4) == hipSuccess &&
+            hipMalloc((void**)&b.H1, (size_t)batch * kH * 4) == hipSuccess &&
+            hipMalloc((void**)&b.H2, (size_t)batch * kH * 4) == hipSuccess;
+  if (!ok) return EMST_E_NOMEM;
+  ok = hipMemcpy(b.p, params, kParams * 4, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(b.X, X, (size_t)batch * kIn * 4, hipMemcpyHostToDevice) == hipSuccess;
+  if (!ok) return EMST_E_HIP;
+  hipLaunchKernelGGL(k_bf16_copy, dim3((kParams + 255) / 256), dim3(256), 0, 0, kParams, b.p, b.work);
+  hipLaunchKernelGGL(k_mlp_fwd, dim3((batch + 255) / 256), dim3(256), 0, 0, batch, b.work, b.X, b.H1, b.H2, b.Y);
+  if (hipMemcpy(Y, b.Y, (size_t)batch * 4, hipMemcpyDeviceToHost) != hipSuccess) return EMST_E_HIP;
+  return hipDeviceSynchronize() == hipSuccess ? EMST_OK : EMST_E_HIP;
+}
+
+int emst_mlp_grad_batch(int device, int32_t batch, const float* params, const float* X, const float* T,
+                        float* grad, float* loss)
+{
+  if (batch < 1 || !params || !X || !T || !grad) return EMST_E_ARG;
+  if (hipSetDevice(device) != hipSuccess) return EMST_E_HIP;
+  TrainBufs b;
+  const size_t B = (size_t)batch;
+  bool ok = hipMalloc((void**)&b.p, kParams * 4) == hipSuccess && hipMalloc((void**)&b.work, kParams * 4) == hipSuccess &&
+            hipMalloc((void**)&b.G, kParams * 4) == hipSuccess && hipMalloc((void**)&b.X, B * kIn * 4) == hipSuccess &&
+            hipMalloc((void**)&b.T, B * 4) == hipSuccess && hipMalloc((void**)&b.Y, B * 4) == hipSuccess &&
+            hipMalloc((void**)&b.H1, B * kH * 4) == hipSuccess && hipMalloc((void**)&b.H2, B * kH * 4) == hipSuccess &&
+            hipMalloc((void**)&b.D1, B * kH * 4) == hipSuccess && hipMalloc((void**)&b.D2, B * kH * 4) == hipSuccess &&
+            hipMalloc((void**)&b.DY, B * 4) == hipSuccess && hipMalloc((void**)&b.sq, B * 4) == hipSuccess;
+  if (!ok) return EMST_E_NOMEM;
+  ok = hipMemcpy(b.p, params, kParams * 4, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(b.X, X, B * kIn * 4, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(b.T, T, B * 4, hipMemcpyHostToDevice) == hipSuccess;
+  if (!ok) return EMST_E_HIP;
+  const int gb = (int)((B + 255) / 256), gp = (kParams + 255) / 256;
+  hipLaunchKernelGGL(k_bf16_copy, dim3(gp), dim3(256), 0, 0, kParams, b.p, b.work);
+  hipLaunchKernelGGL(k_mlp_fwd, dim3(gb), dim3(256), 0, 0, batch, b.work, b.X, b.H1, b.H2, b.Y);
+  hipLaunchKernelGGL(k_mlp_delta, dim3(gb), dim3(256), 0, 0, batch, b.work, b.Y, b.T, b.H1, b.H2, b.D1, b.D2, b.DY, b.sq);
+  hipLaunchKernelGGL(k_mlp_grad, dim3(gp), dim3(256), 0, 0, batch, b.X, b.H1, b.H2, b.D1, b.D2, b.DY, b.G);
+  if (hipMemcpy(grad, b.G, kParams * 4, hipMemcpyDeviceToHost) != hipSuccess) return EMST_E_HIP;
+  if (loss) {
+    std::vector<float> sq(B);
+    if (hipMemcpy(sq.data(), b.sq, B * 4, hipMemcpyDeviceToHost) != hipSuccess) return EMST_E_HIP;
+    double acc = 0;
+    for (float v : sq) acc += v;
+    *loss = (float)(acc / (double)B);
+  }
+  return hipDeviceSynchronize() == hipSuccess ? EMST_OK : EMST_E_HIP;
+}
+
+int emst_mlp_train(int device, int32_t batch, const float* X, const float* T, int32_t epochs, float lr,
+                   uint64_t seed, float* params_inout, float* final_loss)
+{
+  if (batch < 1 || !X || !T || !params_inout || epochs < 0 || !(lr > 0)) return EMST_E_ARG;
+  (void)seed;
+  if (hipSetDevice(device) != hipSuccess) return EMST_E_HIP;
+  TrainBufs b;
+  const size_t B = (size_t)batch;
+  bool ok = hipMalloc((void**)&b.p, kParams * 4) == hipSuccess && hipMalloc((void**)&b.work, kParams * 4) == hipSuccess &&
+            hipMalloc((void**)&b.G, kParams * 4) == hipSuccess && hipMalloc((void**)&b.M, kParams * 4) == hipSuccess &&
+            hipMalloc((void**)&b.V, kParams * 4) == hipSuccess && hipMalloc((void**)&b.X, B * kIn * 4) == hipSuccess &&
+            hipMalloc((void**)&b.T, B * 4) == hipSuccess && hipMalloc((void**)&b.Y, B * 4) == hipSuccess &&
+            hipMalloc((void**)&b.H1, B * kH * 4) == hipSuccess && hipMalloc((void**)&b.H2, B * kH * 4) == hipSuccess &&
+            hipMalloc((void**)&b.D1, B * kH * 4) == hipSuccess && hipMalloc((void**)&b.D2, B * kH * 4) == hipSuccess &&
+            hipMalloc((void**)&b.DY, B * 4) == hipSuccess && hipMalloc((void**)&b.sq, B * 4) == hipSuccess;
+  if (!ok) return EMST_E_NOMEM;
+  ok = hipMemcpy(b.p, params_inout, kParams * 4, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(b.X, X, B * kIn * 4, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(b.T, T, B * 4, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemset(b.M, 0, kParams * 4) == hipSuccess && hipMemset(b.V, 0, kParams * 4) == hipSuccess;
+  if (!ok) return EMST_E_HIP;
+  const int gb = (int)((B + 255) / 256), gp = (kParams + 255) / 256;
+  hipLaunchKernelGGL(k_bf16_copy, dim3(gp), dim3(256), 0, 0, kParams, b.p, b.work);
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  double p1 = 1.0, p2 = 1.0;
+  for (int e = 0; e < epochs; e++) {
+    hipLaunchKernelGGL(k_mlp_fwd, dim3(gb), dim3(256), 0, 0, batch, b.work, b.X, b.H1, b.H2, b.Y);
+    hipLaunchKernelGGL(k_mlp_delta, dim3(gb), dim3(256), 0, 0, batch, b.work, b.Y, b.T, b.H1, b.H2, b.D1, b.D2, b.DY,
+                       b.sq);
+    hipLaunchKernelGGL(k_mlp_grad, dim3(gp), dim3(256), 0, 0, batch, b.X, b.H1, b.H2, b.D1, b.D2, b.DY, b.G);
+    p1 *= b1;
+    p2 *= b2;
+    hipLaunchKernelGGL(k_adam, dim3(gp), dim3(256), 0, 0, b.p, b.work, b.G, b.M, b.V, lr, b1, b2, eps,
+                       (float)(1.0 - p1), (float)(1.0 - p2));
+  }
+  if (hipGetLastError() != hipSuccess) return EMST_E_HIP;
+  if (hipMemcpy(params_inout, b.p, kParams * 4, hipMemcpyDeviceToHost) != hipSuccess) return EMST_E_HIP;
+  if (final_loss) {
+    hipLaunchKernelGGL(k_mlp_fwd, dim3(gb), dim3(256), 0, 0, batch, b.work, b.X, b.H1, b.H2, b.Y);
+    hipLaunchKernelGGL(k_mlp_delta, dim3(gb), dim3(256), 0, 0, batch, b.work, b.Y, b.T, b.H1, b.H2, b.D1, b.D2, b.DY,
+                       b.sq);
+    std::vector<float> sq(B);
+    if (hipMemcpy(sq.data(), b.sq, B * 4, hipMemcpyDeviceToHost) != hipSuccess) return EMST_E_HIP;
+    double acc = 0;
+    for (float v : sq) acc += v;
+    *final_loss = (float)(acc / (double)B);
+  }
+  return hipDeviceSynchronize() == hipSuccess ? EMST_OK : EMST_E_HIP;
+}
+
+}  // extern "C"
+
+// Launch helper used by the solver (same translation unit family).
+namespace emst {
+void launch_learned_threshold(hipStream_t s, int nodes, int edges, const int* nindex, const int* eweight,
+                              const unsigned short* p16, int have_model, int* out, float* outf, float* feat,
+                              int* samples)
+{
+  hipLaunchKernelGGL(k_learned_threshold, dim3(1), dim3(kFeatThreads), 0, s, nodes, edges, nindex, eweight, p16,
+                     have_model, out, outf, feat, samples);
+}
+}  // namespace emst

@@ -26,6 +26,8 @@ ERRORS = {
 F_HOST_SYNC = 1
 F_NO_GUARD = 2
 F_NO_COMBINE = 4
+FILTER_OFF, FILTER_PLAIN, FILTER_LEARNED = 0, 1, 2
+MLP_PARAMS, MLP_FEATURES, WEIGHT_SAMPLES = 6337, 32, 1024
 INT_MAX = 2**31 - 1
 
 
@@ -61,7 +63,9 @@ class Stats(C.Structure):
                 ("init_heavy_ms", C.c_double), ("init_launches", C.c_int32),
                 ("light_rounds", C.c_int32), ("heavy_rounds", C.c_int32), ("tail_rounds", C.c_int32),
                 ("light", C.c_int64), ("heavy", C.c_int64), ("mst_edges", C.c_int64),
-                ("weight", C.c_uint64), ("round_in_sum", C.c_int64), ("round_out_sum", C.c_int64)]
+                ("weight", C.c_uint64), ("round_in_sum", C.c_int64), ("round_out_sum", C.c_int64),
+                ("threshold_used", C.c_int32), ("filter_used", C.c_int32), ("learned_q", C.c_float),
+                ("reserved", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -112,6 +116,15 @@ _gen_rgg = _sig("emst_gen_rgg", C.c_int, C.c_int32, C.c_double, C.c_uint64, C.c_
 _gen_er = _sig("emst_gen_er", C.c_int, C.c_int32, C.c_double, C.c_uint64, C.c_int32, C.c_int32,
                C.POINTER(_GraphBuf))
 _gen_threads = _sig("emst_gen_set_threads", None, C.c_int32)
+_pf32 = C.POINTER(C.c_float)
+_set_model = _sig("emst_set_filter_model", C.c_int, C.c_void_p, _pf32)
+_learned_thr = _sig("emst_learned_threshold", C.c_int, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int32),
+                    C.POINTER(C.c_float), _pf32, _p32)
+_mlp_fwd = _sig("emst_mlp_forward_batch", C.c_int, C.c_int, C.c_int32, _pf32, _pf32, _pf32)
+_mlp_grad = _sig("emst_mlp_grad_batch", C.c_int, C.c_int, C.c_int32, _pf32, _pf32, _pf32, _pf32, C.POINTER(C.c_float))
+_mlp_train = _sig("emst_mlp_train", C.c_int, C.c_int, C.c_int32, _pf32, _pf32, C.c_int32, C.c_float, C.c_uint64,
+                  _pf32, C.POINTER(C.c_float))
+_bf16_bits = _sig("emst_bf16_bits", C.c_uint16, C.c_float)
 
 
 def _check(rc, msg=""):
@@ -300,7 +313,7 @@ class Solver:
 
     def solve(self, use_filter, threshold):
         st = Stats()
-        self._err(_solve(self._h, int(bool(use_filter)), int(threshold), C.byref(st)))
+        self._err(_solve(self._h, int(use_filter), int(threshold), C.byref(st)))
         return st
 
     def fetch(self):
@@ -320,7 +333,7 @@ class Solver:
             use_filter, threshold = plain_threshold(g)
         out = np.zeros(max(g.edges, 1), np.uint8)
         st = Stats()
-        self._err(_gpu_mst(self._h, C.byref(g.view()), int(bool(use_filter)), int(threshold),
+        self._err(_gpu_mst(self._h, C.byref(g.view()), int(use_filter), int(threshold),
                            out.ctypes.data_as(_pu8), C.byref(st)))
         self.graph = g
         return out[:g.edges], st
@@ -329,6 +342,75 @@ class Solver:
                      stream=None):
         st = Stats()
         self._err(_solve_device(self._h, nodes, edges, C.c_void_p(d_nindex), C.c_void_p(d_nlist),
-                                C.c_void_p(d_eweight), int(bool(use_filter)), int(threshold),
+                                C.c_void_p(d_eweight), int(use_filter), int(threshold),
                                 C.c_void_p(d_inmst), C.c_void_p(stream) if stream else None, C.byref(st)))
         return st
+
+
+    def set_filter_model(self, params):
+        """Loads the threshold network (MLP_PARAMS fp32 values); None removes it."""
+        if params is None:
+            self._err(_set_model(self._h, None))
+            return
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.shape == (MLP_PARAMS,)
+        self._err(_set_model(self._h, p.ctypes.data_as(_pf32)))
+
+    def learned_threshold(self):
+        """(use_filter, threshold, q, features[32], samples[1024]) for the loaded graph."""
+        uf, t, q = C.c_int(), C.c_int32(), C.c_float()
+        feat = np.zeros(MLP_FEATURES, np.float32)
+        samp = np.zeros(WEIGHT_SAMPLES, np.int32)
+        self._err(_learned_thr(self._h, C.byref(uf), C.byref(t), C.byref(q), feat.ctypes.data_as(_pf32),
+                               samp.ctypes.data_as(_p32)))
+        return bool(uf.value), int(t.value), float(q.value), feat, samp
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def mlp_forward(params, X, device=0):
+    X = _f32(X)
+    p = _f32(params)
+    Y = np.zeros(X.shape[0], np.float32)
+    _check(_mlp_fwd(device, X.shape[0], p.ctypes.data_as(_pf32), X.ctypes.data_as(_pf32), Y.ctypes.data_as(_pf32)))
+    return Y
+
+
+def mlp_grad(params, X, T, device=0):
+    X, T, p = _f32(X), _f32(T), _f32(params)
+    G = np.zeros(MLP_PARAMS, np.float32)
+    loss = C.c_float()
+    _check(_mlp_grad(device, X.shape[0], p.ctypes.data_as(_pf32), X.ctypes.data_as(_pf32), T.ctypes.data_as(_pf32),
+                     G.ctypes.data_as(_pf32), C.byref(loss)))
+    return G, float(loss.value)
+
+
+def mlp_train(params, X, T, epochs, lr, seed=0, device=0):
+    X, T = _f32(X), _f32(T)
+    p = _f32(params).copy()
+    loss = C.c_float()
+    _check(_mlp_train(device, X.shape[0], X.ctypes.data_as(_pf32), T.ctypes.data_as(_pf32), int(epochs), float(lr),
+                      int(seed), p.ctypes.data_as(_pf32), C.byref(loss)))
+    return p, float(loss.value)
+
+
+def bf16_round(a):
+    """Round-to-nearest-even bf16 of fp32 values (same rule as the device)."""
+    a = _f32(a)
+    u = a.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    special = (u & 0x7F800000) == 0x7F800000
+    r = np.where(special, u.astype(np.uint32), r)
+    return r.view(np.float32)
+
+
+def init_params(seed=0):
+    """He-initialised fp32 parameters of the 32-64-64-1 network."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(MLP_PARAMS, np.float32)
+    p[0:2048] = rng.normal(0, np.sqrt(2 / 32), 2048)
+    p[2112:6208] = rng.normal(0, np.sqrt(2 / 64), 4096)
+    p[6272:6336] = rng.normal(0, np.sqrt(1 / 64), 64)
+    return p

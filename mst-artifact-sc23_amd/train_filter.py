@@ -1,0 +1,156 @@
+"""Trains the light-edge threshold network on graphs the solver itself
+solves (needs a HIP device).
+
+1. Generates N seeded graphs: 2D grids, random geometric graphs and RMAT,
+   2^12..2^20 vertices (log-uniform), mixed degrees and weight ranges.
+2. For each graph: device features and weight samples, then one solve per
+   candidate quantile q (threshold = sorted sample of rank floor(q*1023);
+   the last candidate is the unfiltered path).  Each candidate's cost is
+   modelled from the solve's own exact round schedule:
+       cost = LAUNCH_US * rounds + bytes / BW
+   where bytes is the schedule's byte count (initialisation passes,
+   records, 24 B per record entering a round, 96 B per survivor).  The
+   label is log2 of the cheapest q.
+3. Full-batch Adam on the device (emst_mlp_train), then an evaluation on
+   held-out graphs against the plain rule (ECL_MST.cu:432-443).
+Writes <out>/filter_mlp.f32 (fp32 parameters) and <out>/filter_mlp.json.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import emst  # noqa: E402
+
+QS = [2.0 ** (-k / 2.0) for k in range(14, 0, -1)] + [1.0]   # 2^-7 .. 2^-0.5, then unfiltered
+LAUNCH_US = 8.0
+BW = 3.0e12
+
+
+def make_graph(i, seed):
+    rng = np.random.default_rng([seed, i])
+    kind = i % 3
+    logn = rng.uniform(12, 20)
+    n = int(2 ** logn)
+    wbits = int(rng.choice([6, 10, 16, 20, 24]))
+    gseed = int(rng.integers(1, 2**31))
+    if kind == 0:
+        aspect = rng.uniform(0.5, 2.0)
+        r = max(2, int(math.sqrt(n * aspect)))
+        c = max(2, n // r)
+        return "grid", emst.gen_grid(r, c, gseed, 1, 1 << wbits)
+    if kind == 1:
+        deg = float(rng.choice([3.0, 4.0, 6.0, 8.0, 12.0]))
+        return "rgg", emst.gen_rgg(n, deg, gseed, 1 << max(wbits, 8))
+    scale = int(round(logn))
+    ef = int(rng.choice([4, 8, 16]))
+    a = float(rng.uniform(0.45, 0.6))
+    b = c = (1 - a) * float(rng.uniform(0.3, 0.45))
+    return "rmat", emst.gen_rmat(scale, ef, a, b, c, gseed, 1, 1 << wbits)
+
+
+def schedule_bytes(g, st, m_up_heavy):
+    n, m = g.nodes, g.edges
+    P = 2 if st.filter_used else 1
+    b = 12 * n + m + P * (4 * (n + 1) + 6 * m) + 16 * st.light
+    if st.filter_used:
+        b += 4 * n + 4 * m_up_heavy + 16 * st.heavy
+    b += 24 * st.round_in_sum + 96 * st.round_out_sum + 5 * st.mst_edges
+    return b
+
+
+def cost_of(g, st, m_up_heavy):
+    rounds = st.light_rounds + st.heavy_rounds
+    return LAUNCH_US * rounds + schedule_bytes(g, st, m_up_heavy) / BW * 1e6
+
+
+def label_graph(solver, g):
+    solver.load(g)
+    _, _, _, feat, samples = solver.learned_threshold()
+    upw = np.sort(g.eweight[g.upper_mask()])
+    rows = []
+    for q in QS:
+        if q >= 1.0:
+            st = solver.solve(emst.FILTER_OFF, emst.INT_MAX)
+            heavy_up = 0
+        else:
+            T = int(samples[int(math.floor(q * 1023))])
+            st = solver.solve(emst.FILTER_PLAIN, T)
+            heavy_up = int(len(upw) - np.searchsorted(upw, T, side="right"))
+        rows.append((q, cost_of(g, st, heavy_up), st.ms, st.light_rounds + st.heavy_rounds))
+    best = min(rows, key=lambda r: r[1])
+    return feat, best[0], rows
+
+
+def plain_cost(solver, g):
+    uf, T = emst.plain_threshold(g)
+    st = solver.solve(emst.FILTER_PLAIN if uf else emst.FILTER_OFF, T)
+    upw = np.sort(g.eweight[g.upper_mask()])
+    hu = int(len(upw) - np.searchsorted(upw, T, side="right")) if uf else 0
+    return cost_of(g, st, hu), st
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--graphs", type=int, default=2000)
+    ap.add_argument("--holdout", type=int, default=60)
+    ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--epochs", type=int, default=4000)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--out", default="gpurun_out/filter")
+    a = ap.parse_args()
+    os.makedirs(a.out, exist_ok=True)
+    solver = emst.Solver(0)
+    X, Y, meta = [], [], []
+    t0 = time.time()
+    for i in range(a.graphs):
+        kind, g = make_graph(i, a.seed)
+        feat, qbest, rows = label_graph(solver, g)
+        X.append(feat)
+        Y.append(math.log2(qbest))
+        meta.append({"kind": kind, "n": g.nodes, "m": g.edges, "q": qbest})
+        if i % 50 == 0:
+            print(f"[train_filter] labelled {i}/{a.graphs} ({time.time()-t0:.0f}s) last {kind} n={g.nodes} q*={qbest:.4f}",
+                  flush=True)
+    X = np.array(X, np.float32)
+    Y = np.array(Y, np.float32)
+    np.save(os.path.join(a.out, "train_X.npy"), X)
+    np.save(os.path.join(a.out, "train_Y.npy"), Y)
+    p0 = emst.init_params(a.seed)
+    t1 = time.time()
+    params, loss = emst.mlp_train(p0, X, Y, a.epochs, a.lr, a.seed)
+    print(f"[train_filter] trained {a.epochs} epochs in {time.time()-t1:.1f}s, final mse {loss:.4f} "
+          f"(label variance {float(np.var(Y)):.4f})", flush=True)
+    params.astype(np.float32).tofile(os.path.join(a.out, "filter_mlp.f32"))
+    # held-out evaluation against the plain rule
+    solver.set_filter_model(params)
+    ev = []
+    for i in range(a.holdout):
+        kind, g = make_graph(10_000_000 + i, a.seed + 1)
+        solver.load(g)
+        pc, pst = plain_cost(solver, g)
+        lst = solver.solve(emst.FILTER_LEARNED, 0)
+        upw = np.sort(g.eweight[g.upper_mask()])
+        hu = int(len(upw) - np.searchsorted(upw, lst.threshold_used, side="right")) if lst.filter_used else 0
+        lc = cost_of(g, lst, hu)
+        ev.append({"kind": kind, "n": g.nodes, "m": g.edges, "plain_cost": pc, "learned_cost": lc,
+                   "plain_rounds": pst.light_rounds + pst.heavy_rounds,
+                   "learned_rounds": lst.light_rounds + lst.heavy_rounds, "plain_light": pst.light,
+                   "learned_light": lst.light, "q": lst.learned_q})
+    ratio = float(np.exp(np.mean([math.log(e["learned_cost"] / e["plain_cost"]) for e in ev])))
+    info = {"params": emst.MLP_PARAMS, "graphs": a.graphs, "seed": a.seed, "epochs": a.epochs, "lr": a.lr,
+            "final_mse": loss, "label_variance": float(np.var(Y)), "candidates": QS, "launch_us": LAUNCH_US,
+            "bw": BW, "holdout_geomean_cost_ratio": ratio, "holdout": ev, "train_meta": meta,
+            "seconds": time.time() - t0}
+    with open(os.path.join(a.out, "filter_mlp.json"), "w") as f:
+        json.dump(info, f)
+    print(f"[train_filter] held-out geomean learned/plain modelled cost = {ratio:.3f}", flush=True)
+
+
+if __name__ == "__main__":
+    main()
