@@ -91,13 +91,16 @@ typedef struct {
 
 /* Tunables.  Zero in a field selects the built-in default. */
 typedef struct {
-  int32_t block;        /* threads per workgroup for the round kernels (64..1024)  */
-  int32_t grid_per_cu;  /* resident workgroups per CU for grid-stride kernels      */
-  int32_t tile_edges;   /* CSR edges per wave task in the initialisation kernels   */
-  int32_t stage_recs;   /* LDS staging records per wave before a worklist flush    */
-  int32_t tail_max;     /* worklist size at or below which the one-workgroup tail  */
-                        /* kernel finishes the phase (-1 disables the tail)        */
-  int32_t flags;        /* EMST_F_* bits                                           */
+  int32_t block;            /* threads per workgroup of the round kernels: 256, 512 or 1024 */
+  int32_t grid_per_cu;      /* workgroups per CU of the grid-stride round kernels            */
+  int32_t tile_edges;       /* CSR edges per wave task of the initialisation kernels:        */
+                            /* 256, 512 or 1024 (the edge-centric replacement of the         */
+                            /* reference's thread/warp degree cutoff, ECL_MST.cu:167, 185)   */
+  int32_t stage_recs;       /* LDS compaction staging records per wave before a flush        */
+  int32_t tail_max;         /* worklist size at or below which one workgroup finishes the    */
+                            /* phase (-1 disables the tail)                                  */
+  int32_t flags;            /* EMST_F_* bits                                                 */
+  int32_t init_grid_per_cu; /* workgroups per CU of the initialisation kernels               */
 } emst_config;
 
 #define EMST_F_HOST_SYNC 1  /* read the worklist size back every round, as ECL_MST.cu:318 */
@@ -148,6 +151,14 @@ int emst_load(emst_ctx* ctx, const emst_graph* g);
 int emst_solve(emst_ctx* ctx, int use_filter, int32_t threshold, emst_stats* stats);
 int emst_fetch(emst_ctx* ctx, uint8_t* inMST);
 
+/* Project autotuner (no counterpart in the reference, whose shape is fixed
+ * at compile time, ECL_MST.cu:47-49): coordinate descent over block,
+ * tile_edges, stage_recs, init_grid_per_cu, grid_per_cu and tail_max,
+ * timing `reps` solves of the loaded graph per candidate (median).  The
+ * best configuration is installed on the context and returned. */
+int emst_autotune(emst_ctx* ctx, int use_filter, int32_t threshold, int32_t reps, emst_config* best,
+                  double* best_ms);
+
 /* Device-pointer form of emst_solve for callers that own device memory
  * (e.g. tensors).  All pointers are device pointers on the context's
  * device; d_inMST receives edges bytes.  The CSR is not validated here
@@ -161,6 +172,41 @@ int emst_solve_device(emst_ctx* ctx, int32_t nodes, int32_t edges, const int32_t
  * solve every vertex's root is the smallest vertex id of its tree
  * (join() at ECL_MST.cu:137-144 hooks the larger root under the smaller). */
 int emst_fetch_parent(emst_ctx* ctx, int32_t* parent);
+
+/* ---- batched solving (no counterpart in the reference) --------------------
+ * Many independent graphs packed into HBM arenas and solved together: each
+ * arena is the disjoint union of consecutive graphs with fewer than 2^31
+ * edges, solved with per-graph thresholds; every graph's result is
+ * bit-identical to solving it alone.  With use_graph != 0 the launch
+ * sequence of each arena is captured once in a hipGraph and replayed. */
+typedef struct emst_batch emst_batch;
+typedef struct {
+  double ms;                 /* device time of all arenas                       */
+  int32_t chunks;            /* arenas (union solves)                           */
+  int32_t graphs;
+  int64_t nodes, edges;      /* totals over the batch                           */
+  int64_t light, heavy;      /* records over all arenas                         */
+  int32_t light_rounds;      /* largest per-arena round counts                  */
+  int32_t heavy_rounds;
+  int32_t replays;           /* arenas served by a hipGraph replay              */
+  int32_t filtered_graphs;   /* graphs that used the light/heavy split          */
+} emst_batch_stats;
+/* Validates and uploads the graphs (host arrays stay owned by the caller).
+ * max_arena_edges caps the edges of one arena (0: 2^31-2, the int32
+ * limit); a single graph above the cap gets an arena of its own. */
+int emst_batch_create(emst_ctx* ctx, int32_t ngraphs, const emst_graph* graphs, int64_t max_arena_edges,
+                      emst_batch** out);
+void emst_batch_destroy(emst_batch* b);
+int emst_batch_info(const emst_batch* b, int32_t* chunks, int64_t* total_nodes, int64_t* total_edges);
+/* use_filter: EMST_FILTER_OFF, EMST_FILTER_PLAIN (thresholds[g] per graph,
+ * INT32_MAX for unfiltered; NULL applies the plain rule of
+ * ECL_MST.cu:432-443 per graph) or EMST_FILTER_LEARNED (per-graph network
+ * inference on the device). */
+int emst_batch_solve(emst_batch* b, int use_filter, const int32_t* thresholds, int use_graph, emst_batch_stats* st);
+/* inMST of graph g (its own edge count of bytes). */
+int emst_batch_fetch(emst_batch* b, int32_t g, uint8_t* inMST);
+/* Per-graph thresholds and filter flags of the last solve. */
+int emst_batch_thresholds(const emst_batch* b, int32_t* thresholds, int32_t* use_filter);
 
 /* ---- learned light-edge threshold (no counterpart in the reference) -----
  * A 32-64-64-1 network predicts log2 of the light-edge quantile from 32

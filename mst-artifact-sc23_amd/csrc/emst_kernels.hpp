@@ -20,8 +20,6 @@ namespace emst {
 typedef unsigned long long u64;
 
 constexpr int kWave = 64;
-constexpr int kSub = 8;                       // 64-edge sub-chunks per wave task
-constexpr int kGroupEdges = kWave * kSub;     // CSR edges per wave task
 constexpr u64 kNoKey = ~0ull;                 // ECL_MST.cu:152, 248
 
 // Indices of the device statistics array.

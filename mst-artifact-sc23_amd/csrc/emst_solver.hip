@@ -2,60 +2,63 @@
 // semantics of ECL-MST's gpuMST path (ECL-MST/ECL_MST.cu:127-362).
 //
 // Device flow of one solve (the reference's timed region, ECL_MST.cu:301-345):
-//   k_setup          parent[v]=v, minv[v]=max, per-task start vertices
-//                    (initPM, ECL_MST.cu:147-154), inMST cleared by k_zero
-//   k_init_wl<light> records {v, u, w, j} of upper-direction edges with w<=T
-//                    (initWL<true>, ECL_MST.cu:157-207)
-//   rounds           k_select: drop records inside one component, write back
-//                    representatives, guarded 64-bit atomicMin of the key on
-//                    both representatives (kernel1, ECL_MST.cu:210-226)
-//                    k_join: the record holding a representative's minimum
-//                    key clears that minimum, hooks, and marks inMST[j]
-//                    (kernel2 + kernel3, ECL_MST.cu:229-251, fused)
-//   k_flatten        parent[v] = root(v) before the heavy pass
-//   k_init_wl<heavy> heavy upper-direction edges whose roots differ
-//                    (initWL<false>, ECL_MST.cu:157-207, launch :326)
-//   rounds           as above
+//   k_setup            parent[v]=v, minv[v]=max, per-task start vertices
+//                      (initPM, ECL_MST.cu:147-154); inMST cleared by k_zero
+//   k_init_wl<light>   records {v, u, w, j} of upper-direction edges with w<=T
+//                      (initWL<true>, ECL_MST.cu:157-207)
+//   k_select<first>    the phase's first kernel1 round: every endpoint is a
+//                      root and every record crosses, so it only offers keys
+//   rounds             k_join: the record holding a root's minimum key
+//                      clears that minimum, hooks, and marks inMST[j]
+//                      (kernel2 + kernel3, ECL_MST.cu:229-251, fused)
+//                      k_select: drop records inside one component, write
+//                      back roots, offer keys (kernel1, ECL_MST.cu:210-226)
+//   k_pick_big, k_flatten  parent[v] = root(v); bitmap of one big component
+//   k_init_wl<heavy>   heavy upper-direction edges whose roots differ
+//                      (initWL<false>, ECL_MST.cu:157-207, launch :326)
+//   rounds             as above
 //
 // The host does not read the worklist size back every round (the reference
 // does, ECL_MST.cu:318).  Round kernels run as fixed grid-stride grids that
-// read sizes from device memory; k_join posts the surviving count into
+// read sizes from device memory; k_join posts the count it joins into
 // host-mapped memory and the host stops enqueueing rounds when it reads 0,
-// a couple of rounds behind the device.  When a round's survivors fit
-// cfg.tail_max, workgroup 0 of k_join finishes the whole phase with
-// workgroup barriers instead of further launches.
+// a couple of rounds behind the device.  When a list fits cfg.tail_max,
+// workgroup 0 of k_join finishes the whole phase with workgroup barriers.
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
-#include <cstdio>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/emst.h"
+#include "emst_internal.hpp"
 #include "emst_kernels.hpp"
-#include "learned.hpp"
 #include "host_util.hpp"
+#include "learned.hpp"
 
 using namespace emst;
 
 namespace {
 
-constexpr int kInitWaves = 4;        // waves per workgroup in k_init_wl
-constexpr int kRoundWaves = 8;       // waves per workgroup in k_select/k_join
-constexpr int kHflagSlots = 512;     // host-visible round results per solve
+constexpr int kInitWaves = 4;     // waves per workgroup in k_init_wl
+constexpr int kHflagSlots = 512;  // host-visible round results per solve
+constexpr int kSelItems = 4;      // records per thread per k_select tile
+constexpr int kTbl = 1024;        // LDS min-combining table entries
 
 // ---------------------------------------------------------------------------
 // k_setup: initPM (ECL_MST.cu:147-154) plus the start vertex of every
-// kGroupEdges-edge task: the vertex v whose adjacency range holds the task's
+// task_edges-edge task: the vertex v whose adjacency range holds the task's
 // first edge writes it.  Zero-degree vertices own no edge and write nothing.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_setup(int nodes, const int* __restrict__ nindex,
                                                int* __restrict__ parent, u64* __restrict__ minv,
-                                               int* __restrict__ gsrc, int* __restrict__ cnt,
+                                               int* __restrict__ gsrc, int task_edges, int* __restrict__ cnt,
                                                long long* __restrict__ stats)
 {
   const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -69,8 +72,8 @@ __global__ __launch_bounds__(256) void k_setup(int nodes, const int* __restrict_
     minv[v] = kNoKey;
     const int beg = nindex[v], end = nindex[v + 1];
     if (end > beg) {
-      const long long g0 = ((long long)beg + kGroupEdges - 1) / kGroupEdges;
-      const long long g1 = ((long long)end + kGroupEdges - 1) / kGroupEdges;
+      const long long g0 = ((long long)beg + task_edges - 1) / task_edges;
+      const long long g1 = ((long long)end + task_edges - 1) / task_edges;
       for (long long g = g0; g < g1; g++) gsrc[g] = (int)v;
     }
   }
@@ -105,46 +108,52 @@ __global__ __launch_bounds__(256) void k_zero(uint8_t* __restrict__ p, long long
 //                one large component (2 MB for 2^24 vertices, resident in
 //                L2); an edge with both ends marked is skipped without
 //                touching the root array.
-// One wave task covers kGroupEdges consecutive CSR positions.  Lane i takes
+// One wave task covers SUB*64 consecutive CSR positions.  Lane i takes
 // positions base+i, base+64+i, ... so every load is one coalesced 256-byte
-// request, and all kSub loads of a task are issued before the first use.
+// request, and all SUB loads of a task are issued before the first use.
 // The source vertex of a position is found against a window of 64 vertex
-// boundaries held one per lane: src(p) = max{v : nindex[v] <= p}.
+// boundaries held one per lane: src(p) = max{v : nindex[v] <= p}.  This
+// edge-centric split replaces the reference's per-vertex thread/warp split
+// at degree 4 (ECL_MST.cu:167, 185): every wave streams whole 256-byte
+// lines whatever the degrees, and hub vertices spread over many waves.
+// T comes from the argument, from dthr (decided on the device), or per
+// source vertex from gthr[vgid[v]] when several graphs are solved together.
 // ---------------------------------------------------------------------------
-template <bool HEAVY>
+template <bool HEAVY, int SUB>
 __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
     const int* __restrict__ nindex, const int* __restrict__ nlist, const int* __restrict__ eweight,
     const int* __restrict__ gsrc, int nodes, int edges, int T, const int* __restrict__ label,
     const unsigned* __restrict__ bigmap, int4* __restrict__ wl, int* __restrict__ cnt, int stage_cap,
-    long long* __restrict__ stats, const int* __restrict__ dthr)
+    long long* __restrict__ stats, const int* __restrict__ dthr, const int* __restrict__ vgid,
+    const int* __restrict__ gthr)
 {
+  constexpr int TE = SUB * kWave;
   extern __shared__ int4 lds_stage[];
-  // dthr: threshold and filter flag decided on the device (learned mode)
   if (dthr) {
-    if (HEAVY && dthr[1] == 0) return;
+    if (HEAVY && dthr[1] == 0) return;  // no heavy phase when unfiltered
     T = dthr[0];
   }
   const int lane = lane_id();
   const int wid = threadIdx.x / kWave;
   Stage st{lds_stage + wid * stage_cap, stage_cap, 0};
-  const long long ngroups = ((long long)edges + kGroupEdges - 1) / kGroupEdges;
+  const long long ngroups = ((long long)edges + TE - 1) / TE;
   const long long nwaves = (long long)gridDim.x * kInitWaves;
   long long emitted = 0;
   for (long long g = (long long)blockIdx.x * kInitWaves + wid; g < ngroups; g += nwaves) {
-    const long long gbeg = g * kGroupEdges;
-    int nl[kSub], w[kSub];
+    const long long gbeg = g * TE;
+    int nl[SUB], w[SUB];
     int base = gsrc[g];
 #pragma unroll
-    for (int u = 0; u < kSub; u++) {
+    for (int u = 0; u < SUB; u++) {
       const long long p = gbeg + u * kWave + lane;
       nl[u] = p < edges ? nlist[p] : -1;
       if (!HEAVY) w[u] = p < edges ? eweight[p] : 0;
     }
     long long bi = (long long)base + 1 + lane;
     int bnd = bi <= nodes ? nindex[bi] : INT_MAX;
-    int src[kSub];
+    int src[SUB];
 #pragma unroll
-    for (int u = 0; u < kSub; u++) {
+    for (int u = 0; u < SUB; u++) {
       const long long pl = gbeg + u * kWave + lane;
       const int p = pl < edges ? (int)pl : INT_MAX - 1;
       bool done = pl >= edges;
@@ -165,16 +174,18 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
     }
     if (!HEAVY) {
 #pragma unroll
-      for (int u = 0; u < kSub; u++) {
+      for (int u = 0; u < SUB; u++) {
         const int j = (int)(gbeg + u * kWave + lane);
-        const bool keep = nl[u] > src[u] && w[u] <= T;
+        const bool up = nl[u] > src[u];
+        const int t = (gthr && up) ? gthr[vgid[src[u]]] : T;
+        const bool keep = up && w[u] <= t;
         emitted += keep;
         stage_push(st, keep, make_int4(src[u], nl[u], w[u], j), wl, cnt);
       }
     } else {
-      bool both[kSub];
+      bool both[SUB];
 #pragma unroll
-      for (int u = 0; u < kSub; u++) {
+      for (int u = 0; u < SUB; u++) {
         both[u] = true;
         if (nl[u] > src[u]) {
           const unsigned bu = bigmap[(unsigned)nl[u] >> 5] >> (nl[u] & 31);
@@ -182,9 +193,9 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
           both[u] = (bu & bv & 1u) != 0;
         }
       }
-      int a[kSub], b[kSub];
+      int a[SUB], b[SUB];
 #pragma unroll
-      for (int u = 0; u < kSub; u++) {
+      for (int u = 0; u < SUB; u++) {
         a[u] = 0;
         b[u] = 0;
         if (!both[u]) {
@@ -193,7 +204,7 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
         }
       }
 #pragma unroll
-      for (int u = 0; u < kSub; u++) {
+      for (int u = 0; u < SUB; u++) {
         const long long p = gbeg + u * kWave + lane;
         const bool keep = !both[u] && a[u] != b[u];
         const int wt = keep ? eweight[p] : 0;
@@ -209,8 +220,10 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
 // Picks the component that the heavy pass' bitmap marks: the most frequent
 // root among 64 evenly spaced vertices (one wave).  Any choice is correct;
 // a frequent one saves the most root lookups.
-__global__ __launch_bounds__(64) void k_pick_big(int nodes, const int* __restrict__ parent, int* __restrict__ out)
+__global__ __launch_bounds__(64) void k_pick_big(int nodes, const int* __restrict__ parent, int* __restrict__ out,
+                                                 const int* __restrict__ dthr)
 {
+  if (dthr && dthr[1] == 0) return;
   const int lane = lane_id();
   const int v = (int)(((long long)lane * nodes) / kWave + ((long long)nodes / (2 * kWave)));
   const int r = find_root(v < nodes ? v : nodes - 1, parent);
@@ -224,27 +237,35 @@ __global__ __launch_bounds__(64) void k_pick_big(int nodes, const int* __restric
   if (lane == 0) out[0] = big;
 }
 
-// ---------------------------------------------------------------------------
-// k_select (kernel1, ECL_MST.cu:210-226): for each record find both roots;
-// records inside one component are dropped, the rest are written back with
-// their roots (implicit path compression) and offer their key to both roots.
-//
-// Two MI355X-specific changes, both invisible in the results:
-//  * Survivors of a 2048-record tile are compacted with a workgroup scan and
-//    reserved with one atomicAdd per tile (a per-record or per-wave counter
-//    atomic serialises on one address).
-//  * Keys are first min-combined per workgroup in an LDS table keyed by
-//    root (4-probe open addressing, claimed slots are never released).  A hub
-//    component that appears in most records is claimed early and then
-//    costs one global atomicMin per workgroup instead of one per record;
-//    device-scope atomics and coherent loads are resolved beyond the
-//    per-XCD L2, so a hot address otherwise serialises the whole grid.
-//    Roots that find no slot fall back to the guarded global atomic.
-// ---------------------------------------------------------------------------
-constexpr int kSelItems = 4;
-constexpr int kSelTile = kRoundWaves * kWave * kSelItems;
-constexpr int kTbl = 1024;
+// parent[v] = root(v), and bit v of bigmap = (root(v) == *big).  Concurrent
+// writers only shorten chains toward the same root, so readers stay correct.
+__global__ __launch_bounds__(256) void k_flatten(int nodes, int* __restrict__ parent, int* __restrict__ cnt,
+                                                 const int* __restrict__ big, unsigned* __restrict__ bigmap,
+                                                 const int* __restrict__ dthr)
+{
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v == 0) {
+    cnt[0] = 0;
+    cnt[1] = 0;
+  }
+  if (dthr && dthr[1] == 0) return;  // unfiltered: no heavy phase
+  const int bigroot = *big;
+  int r = -1;
+  if (v < nodes) {
+    r = find_root((int)v, parent);
+    parent[v] = r;
+  }
+  const u64 m = __ballot(r == bigroot && r >= 0);
+  const long long v0 = v - lane_id();
+  const long long words = ((long long)nodes + 31) / 32;
+  if (lane_id() == 0 && v0 < nodes) {
+    bigmap[v0 >> 5] = (unsigned)m;
+    if ((v0 >> 5) + 1 < words) bigmap[(v0 >> 5) + 1] = (unsigned)(m >> 32);
+  }
+}
 
+// Offers key to rep: min-combined in the workgroup's LDS table when the rep
+// holds (or claims) a slot within 4 probes, else a guarded global atomicMin.
 __device__ __forceinline__ void offer_key(int* tags, u64* keys, u64* minv, int rep, u64 key, bool guard,
                                           bool combine)
 {
@@ -267,16 +288,35 @@ __device__ __forceinline__ void offer_key(int* tags, u64* keys, u64* minv, int r
   min_key(minv, rep, key, guard);
 }
 
-template <bool FIRST>
-__global__ __launch_bounds__(kRoundWaves * kWave) void k_select(
+// ---------------------------------------------------------------------------
+// k_select (kernel1, ECL_MST.cu:210-226): for each record find both roots;
+// records inside one component are dropped, the rest are written back with
+// their roots (implicit path compression) and offer their key to both roots.
+//
+// Two MI355X-specific changes, both invisible in the results:
+//  * Survivors of a WAVES*256-record tile are compacted with a workgroup
+//    scan and reserved with one atomicAdd per tile (a per-record or per-wave
+//    counter atomic serialises on one address: ~12 ns each on MI355X).
+//  * Keys are first min-combined per workgroup in an LDS table keyed by
+//    root (4-probe open addressing, claimed slots are never released).  A hub
+//    component that appears in most records is claimed early and then
+//    costs one global atomicMin per workgroup instead of one per record.
+// FIRST: the phase's first round; every endpoint is a root and every record
+// crosses (both guaranteed by the initialisation), so finds and the copy
+// are skipped and the list is joined in place.
+// ---------------------------------------------------------------------------
+template <bool FIRST, int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void k_select(
     const int4* __restrict__ wl_in, const int* __restrict__ cnt_in, int4* __restrict__ wl_out,
     int* __restrict__ cnt_out, const int* __restrict__ parent, u64* __restrict__ minv, int guard, int combine,
     long long* __restrict__ stats, int rounds_slot)
 {
+  constexpr int BT = WAVES * kWave;
+  constexpr int TILE = BT * kSelItems;
   __shared__ int s_tags[kTbl];
   __shared__ u64 s_keys[kTbl];
-  __shared__ int s_wtot[kRoundWaves];
-  __shared__ int s_woff[kRoundWaves];
+  __shared__ int s_wtot[WAVES];
+  __shared__ int s_woff[WAVES];
   __shared__ int s_base;
   const int n_in = *cnt_in;
   if (n_in == 0) return;
@@ -285,32 +325,29 @@ __global__ __launch_bounds__(kRoundWaves * kWave) void k_select(
     stats[rounds_slot] += 1;
     stats[ST_IN_SUM] += n_in;
   }
-  for (int i = tid; i < kTbl; i += blockDim.x) {
+  for (int i = tid; i < kTbl; i += BT) {
     s_tags[i] = -1;
     s_keys[i] = kNoKey;
   }
   __syncthreads();
   const int lane = lane_id();
   const int wid = tid / kWave;
-  const long long ntiles = ((long long)n_in + kSelTile - 1) / kSelTile;
+  const long long ntiles = ((long long)n_in + TILE - 1) / TILE;
   for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const long long base = t * kSelTile;
+    const long long base = t * TILE;
     int4 r[kSelItems];
 #pragma unroll
     for (int k = 0; k < kSelItems; k++) {
-      const long long i = base + k * (kRoundWaves * kWave) + tid;
+      const long long i = base + k * BT + tid;
       r[k] = i < n_in ? wl_in[i] : make_int4(0, 0, 0, 0);
     }
     bool keep[kSelItems];
     int mine = 0;
 #pragma unroll
     for (int k = 0; k < kSelItems; k++) {
-      const long long i = base + k * (kRoundWaves * kWave) + tid;
+      const long long i = base + k * BT + tid;
       keep[k] = false;
       if (i < n_in) {
-        // In a phase's first round every endpoint is a root and every
-        // record crosses (initialisation guarantees both), so the finds
-        // and the copy are skipped: the list is joined in place.
         const int a = FIRST ? r[k].x : find_root(r[k].x, parent);
         const int c = FIRST ? r[k].y : find_root(r[k].y, parent);
         if (a != c) {
@@ -336,7 +373,7 @@ __global__ __launch_bounds__(kRoundWaves * kWave) void k_select(
     __syncthreads();
     if (tid == 0) {
       int acc = 0;
-      for (int w = 0; w < kRoundWaves; w++) {
+      for (int w = 0; w < WAVES; w++) {
         s_woff[w] = acc;
         acc += s_wtot[w];
       }
@@ -352,13 +389,13 @@ __global__ __launch_bounds__(kRoundWaves * kWave) void k_select(
   // publish the workgroup's combined minima (after every offer of every
   // wave: the first-round pass has no per-tile barrier)
   __syncthreads();
-  for (int i = tid; i < kTbl; i += blockDim.x) {
+  for (int i = tid; i < kTbl; i += BT) {
     const int rep = s_tags[i];
     if (rep >= 0) min_key(minv, rep, s_keys[i], guard);
   }
 }
 
-// One workgroup finishes a phase: join the current survivors, then repeat
+// One workgroup finishes a phase: join the current list, then repeat
 // select/join until no record crosses components.  All shared state is
 // accessed at device scope; workgroup barriers order the steps.
 __device__ void tail_rounds(int4* cur, int n, int4* nxt, int* parent, u64* minv, uint8_t* inMST,
@@ -425,11 +462,11 @@ __device__ void tail_rounds(int4* cur, int n, int4* nxt, int* parent, u64* minv,
 // with a reset sees either the winner's key or the sentinel, neither equal
 // to its own key, so the outcome is the same as with a separate kernel3.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kRoundWaves * kWave) void k_join(
-    int4* __restrict__ wl, int* __restrict__ cnt, int4* __restrict__ wl_other,
-    int* __restrict__ cnt_other, int* __restrict__ parent, u64* __restrict__ minv,
-    uint8_t* __restrict__ inMST, int tail_max, int* __restrict__ hflag, long long* __restrict__ stats,
-    int rounds_slot)
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void k_join(
+    int4* __restrict__ wl, int* __restrict__ cnt, int4* __restrict__ wl_other, int* __restrict__ cnt_other,
+    int* __restrict__ parent, u64* __restrict__ minv, uint8_t* __restrict__ inMST, int tail_max,
+    int* __restrict__ hflag, long long* __restrict__ stats, int rounds_slot)
 {
   const int n = *cnt;
   if (n > 0 && n <= tail_max) {
@@ -466,33 +503,6 @@ __global__ __launch_bounds__(kRoundWaves * kWave) void k_join(
   }
 }
 
-// parent[v] = root(v), and bit v of bigmap = (root(v) == *big).  Concurrent
-// writers only shorten chains toward the same root, so readers stay correct.
-__global__ __launch_bounds__(256) void k_flatten(int nodes, int* __restrict__ parent, int* __restrict__ cnt,
-                                                 const int* __restrict__ big, unsigned* __restrict__ bigmap,
-                                                 const int* __restrict__ dthr)
-{
-  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v == 0) {
-    cnt[0] = 0;
-    cnt[1] = 0;
-  }
-  if (dthr && dthr[1] == 0) return;  // unfiltered: no heavy phase
-  const int bigroot = *big;
-  int r = -1;
-  if (v < nodes) {
-    r = find_root((int)v, parent);
-    parent[v] = r;
-  }
-  const u64 m = __ballot(r == bigroot && r >= 0);
-  const long long v0 = v - lane_id();
-  const long long words = ((long long)nodes + 31) / 32;
-  if (lane_id() == 0 && v0 < nodes) {
-    bigmap[v0 >> 5] = (unsigned)m;
-    if ((v0 >> 5) + 1 < words) bigmap[(v0 >> 5) + 1] = (unsigned)(m >> 32);
-  }
-}
-
 // Host-sync ablation: a plain kernel2 without the fused tail.
 __global__ __launch_bounds__(256) void k_join_plain(const int4* __restrict__ wl, int n, int* __restrict__ parent,
                                                     u64* __restrict__ minv, uint8_t* __restrict__ inMST)
@@ -513,111 +523,116 @@ __global__ __launch_bounds__(256) void k_join_plain(const int4* __restrict__ wl,
   }
 }
 
+// ---- runtime selection of template instances -------------------------------
+typedef void (*InitFn)(const int*, const int*, const int*, const int*, int, int, int, const int*,
+                       const unsigned*, int4*, int*, int, long long*, const int*, const int*, const int*);
+typedef void (*SelFn)(const int4*, const int*, int4*, int*, const int*, u64*, int, int, long long*, int);
+typedef void (*JoinFn)(int4*, int*, int4*, int*, int*, u64*, uint8_t*, int, int*, long long*, int);
+
+InitFn init_fn(bool heavy, int sub)
+{
+  if (heavy) {
+    if (sub == 4) return k_init_wl<true, 4>;
+    if (sub == 16) return k_init_wl<true, 16>;
+    return k_init_wl<true, 8>;
+  }
+  if (sub == 4) return k_init_wl<false, 4>;
+  if (sub == 16) return k_init_wl<false, 16>;
+  return k_init_wl<false, 8>;
+}
+
+SelFn sel_fn(bool first, int waves)
+{
+  if (first) {
+    if (waves == 4) return k_select<true, 4>;
+    if (waves == 16) return k_select<true, 16>;
+    return k_select<true, 8>;
+  }
+  if (waves == 4) return k_select<false, 4>;
+  if (waves == 16) return k_select<false, 16>;
+  return k_select<false, 8>;
+}
+
+JoinFn join_fn(int waves)
+{
+  if (waves == 4) return k_join<4>;
+  if (waves == 16) return k_join<16>;
+  return k_join<8>;
+}
+
 }  // namespace
 
 // ===========================================================================
 // Host side
 // ===========================================================================
-struct emst_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  int cus = 0;
-  size_t lds_per_block = 0;
-  emst_config cfg{};
-  std::string err;
-  // graph owned by the context (emst_load)
-  int32_t n = 0, m = 0;
-  long long m_up = -1;
-  bool loaded = false;
-  int* d_nindex = nullptr;
-  int* d_nlist = nullptr;
-  int* d_eweight = nullptr;
-  uint8_t* d_inMST = nullptr;
-  size_t cap_nindex = 0, cap_edges = 0;
-  // work buffers, grown on demand and reused
-  int* d_parent = nullptr;
-  u64* d_minv = nullptr;
-  unsigned* d_bigmap = nullptr;
-  size_t cap_nodes = 0;
-  int4* d_wl[2] = {nullptr, nullptr};
-  size_t cap_wl = 0;
-  int* d_gsrc = nullptr;
-  size_t cap_gsrc = 0;
-  int* d_cnt = nullptr;
-  long long* d_stats = nullptr;
-  int* h_flag = nullptr;
-  int* d_flag = nullptr;
-  hipEvent_t ev[8] = {};
-  int32_t last_nodes = 0;
-  // learned threshold
-  unsigned short* d_mlp16 = nullptr;  // bf16 parameters
-  int have_model = 0;
-  int* d_thr = nullptr;               // [threshold, use_filter] decided on the device
-  float* d_q = nullptr;
-  float* d_feat = nullptr;
-  int* d_samples = nullptr;
-};
+namespace emst {
 
-namespace {
-
-int fail(emst_ctx* c, int code, const std::string& what)
+int ctx_fail(emst_ctx* c, int code, const std::string& what)
 {
   if (c) c->err = what;
   return code;
 }
 
+}  // namespace emst
+
+namespace {
+
 int hip_fail(emst_ctx* c, hipError_t e, const char* what)
 {
   char buf[256];
   std::snprintf(buf, sizeof(buf), "HIP error %d in %s: %s", (int)e, what, hipGetErrorString(e));
-  return fail(c, EMST_E_HIP, buf);
+  return ctx_fail(c, EMST_E_HIP, buf);
 }
 
-#define HIPCHK(call)                                   \
-  do {                                                 \
-    hipError_t e_ = (call);                            \
+#define HIPCHK(call)                                       \
+  do {                                                     \
+    hipError_t e_ = (call);                                \
     if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
   } while (0)
 
 emst_config default_config()
 {
   emst_config c{};
-  c.block = kRoundWaves * kWave;
+  c.block = 512;
   c.grid_per_cu = 4;
-  c.tile_edges = kGroupEdges;
+  c.tile_edges = 512;
   c.stage_recs = 512;
   c.tail_max = 4096;
   c.flags = 0;
+  c.init_grid_per_cu = 8;
   return c;
 }
 
 }  // namespace
 
+namespace emst {
+
 // The work buffers are managed in one function so their capacities stay in step.
-static int ensure_buffers(emst_ctx* ctx, int32_t nodes, int32_t edges, long long wl_cap)
+int ensure_buffers(emst_ctx* ctx, int32_t nodes, int32_t edges, long long wl_cap, int task_edges)
 {
-  if ((size_t)nodes > ctx->cap_nodes || !ctx->d_parent || !ctx->d_minv) {
-    if (ctx->d_parent) (void)hipFree(ctx->d_parent);
-    if (ctx->d_minv) (void)hipFree(ctx->d_minv);
-    if (ctx->d_bigmap) (void)hipFree(ctx->d_bigmap);
-    ctx->d_parent = nullptr;
-    ctx->d_minv = nullptr;
-    ctx->d_bigmap = nullptr;
+  if ((size_t)nodes > ctx->cap_nodes || !ctx->d_parent) {
+    for (void** p : {(void**)&ctx->d_parent, (void**)&ctx->d_minv, (void**)&ctx->d_bigmap}) {
+      if (*p) (void)hipFree(*p);
+      *p = nullptr;
+    }
     ctx->cap_nodes = 0;
-    if (hipMalloc((void**)&ctx->d_parent, (size_t)std::max(nodes, 1) * 4) != hipSuccess ||
-        hipMalloc((void**)&ctx->d_minv, (size_t)std::max(nodes, 1) * 8) != hipSuccess ||
-        hipMalloc((void**)&ctx->d_bigmap, ((size_t)std::max(nodes, 1) + 63) / 32 * 4) != hipSuccess)
-      return fail(ctx, EMST_E_NOMEM, "hipMalloc of parent/minv failed");
-    ctx->cap_nodes = (size_t)std::max(nodes, 1);
+    const size_t n = (size_t)std::max(nodes, 1);
+    if (hipMalloc((void**)&ctx->d_parent, n * 4) != hipSuccess || hipMalloc((void**)&ctx->d_minv, n * 8) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_bigmap, (n + 63) / 32 * 4) != hipSuccess)
+      return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of parent/minv failed");
+    ctx->cap_nodes = n;
+    ctx->buf_gen++;
   }
-  const size_t groups = ((size_t)edges + kGroupEdges - 1) / kGroupEdges;
+  const size_t groups = ((size_t)edges + task_edges - 1) / task_edges;
   if (groups > ctx->cap_gsrc || !ctx->d_gsrc) {
     if (ctx->d_gsrc) (void)hipFree(ctx->d_gsrc);
     ctx->d_gsrc = nullptr;
     ctx->cap_gsrc = 0;
-    if (hipMalloc((void**)&ctx->d_gsrc, std::max<size_t>(groups, 1) * 4) != hipSuccess)
-      return fail(ctx, EMST_E_NOMEM, "hipMalloc of task start vertices failed");
-    ctx->cap_gsrc = std::max<size_t>(groups, 1);
+    const size_t want = std::max<size_t>(groups, 1) * 4;  // room for smaller task sizes later
+    if (hipMalloc((void**)&ctx->d_gsrc, want * 4) != hipSuccess)
+      return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of task start vertices failed");
+    ctx->cap_gsrc = want;
+    ctx->buf_gen++;
   }
   if ((size_t)wl_cap > ctx->cap_wl || !ctx->d_wl[0]) {
     for (int i = 0; i < 2; i++) {
@@ -630,97 +645,108 @@ static int ensure_buffers(emst_ctx* ctx, int32_t nodes, int32_t edges, long long
       if (hipMalloc((void**)&ctx->d_wl[i], want * sizeof(int4)) != hipSuccess) {
         char buf[128];
         std::snprintf(buf, sizeof(buf), "hipMalloc of a %zu-record worklist failed", want);
-        return fail(ctx, EMST_E_NOMEM, buf);
+        return ctx_fail(ctx, EMST_E_NOMEM, buf);
       }
     ctx->cap_wl = want;
+    ctx->buf_gen++;
   }
   return EMST_OK;
 }
 
-static int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
-                     const int* d_eweight, int use_filter, int32_t threshold, uint8_t* d_inMST,
-                     hipStream_t s, long long wl_cap, emst_stats* stats)
+}  // namespace emst
+
+namespace {
+}  // namespace
+
+namespace emst {
+
+int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
+              const int* d_eweight, int use_filter, int32_t threshold, uint8_t* d_inMST, hipStream_t s,
+              long long wl_cap, const VertexThresholds* vt, emst_stats* stats, FixedSchedule* fixed)
 {
-  if (nodes < 1 || edges < 0) return fail(ctx, EMST_E_ARG, "node or edge count too low");
-  int rc = ensure_buffers(ctx, nodes, edges, wl_cap);
-  if (rc) return rc;
+  if (nodes < 1 || edges < 0) return ctx_fail(ctx, EMST_E_ARG, "node or edge count too low");
+  if (use_filter < 0 || use_filter > EMST_FILTER_LEARNED) return ctx_fail(ctx, EMST_E_ARG, "bad filter mode");
   const emst_config& cfg = ctx->cfg;
-  const bool learned = use_filter == EMST_FILTER_LEARNED;
-  if (use_filter < 0 || use_filter > EMST_FILTER_LEARNED) return fail(ctx, EMST_E_ARG, "bad filter mode");
+  const int task_edges = cfg.tile_edges;
+  const int sub = task_edges / kWave;
+  const int waves = cfg.block / kWave;
+  int rc = ensure_buffers(ctx, nodes, edges, wl_cap, task_edges);
+  if (rc) return rc;
+  const bool learned = use_filter == EMST_FILTER_LEARNED && !vt;
   const int T = use_filter == EMST_FILTER_PLAIN ? threshold : INT_MAX;
   const int* dthr = learned ? ctx->d_thr : nullptr;
   const bool host_sync = (cfg.flags & EMST_F_HOST_SYNC) != 0;
+  if (host_sync && (learned || vt || fixed))
+    return ctx_fail(ctx, EMST_E_ARG, "host-sync mode takes one graph and a given threshold");
   const int guard = (cfg.flags & EMST_F_NO_GUARD) ? 0 : 1;
   const int combine = (cfg.flags & EMST_F_NO_COMBINE) ? 0 : 1;
   const int tail_max = cfg.tail_max < 0 ? 0 : cfg.tail_max;
   const int round_grid = std::max(1, ctx->cus * std::max(1, cfg.grid_per_cu));
-  const int init_grid = std::max(1, ctx->cus * 8);
+  const int init_grid = std::max(1, ctx->cus * std::max(1, cfg.init_grid_per_cu));
   const int stage_cap = std::max(64, cfg.stage_recs);
   const size_t init_lds = (size_t)kInitWaves * stage_cap * sizeof(int4);
-  if (init_lds > ctx->lds_per_block) return fail(ctx, EMST_E_ARG, "stage_recs exceeds the LDS of a workgroup");
+  if (init_lds > ctx->lds_per_block) return ctx_fail(ctx, EMST_E_ARG, "stage_recs exceeds the LDS of a workgroup");
+  const int nphases = (vt ? vt->any_filter : use_filter != EMST_FILTER_OFF) ? 2 : 1;
+  const int* vgid = vt ? vt->vgid : nullptr;
+  const int* gthr = vt ? vt->gthr : nullptr;
   int* d_cnt = ctx->d_cnt;
   long long* d_stats = ctx->d_stats;
   int4** wl = ctx->d_wl;
   ctx->last_nodes = nodes;
+  const SelFn sel_first = sel_fn(true, waves), sel = sel_fn(false, waves);
+  const JoinFn join = join_fn(waves);
 
-  HIPCHK(hipEventRecord(ctx->ev[0], s));
+  if (!fixed) HIPCHK(hipEventRecord(ctx->ev[0], s));
   if (learned)
-    launch_learned_threshold(s, nodes, edges, d_nindex, d_eweight, ctx->d_mlp16, ctx->have_model, ctx->d_thr,
-                             ctx->d_q, ctx->d_feat, ctx->d_samples);
+    launch_learned_threshold(s, nodes, edges, d_nindex, d_eweight, ctx->d_mlp16, ctx->have_model, ctx->d_thr, ctx->d_q,
+                             ctx->d_feat, ctx->d_samples);
   {
     const int blocks = (int)(((long long)nodes + 255) / 256);
     hipLaunchKernelGGL(k_setup, dim3(blocks), dim3(256), 0, s, nodes, d_nindex, ctx->d_parent, ctx->d_minv,
-                       ctx->d_gsrc, d_cnt, d_stats);
+                       ctx->d_gsrc, task_edges, d_cnt, d_stats);
     const long long z16 = ((long long)edges + 15) / 16;
     const int zb = (int)std::min<long long>(std::max<long long>((z16 + 255) / 256, 1), (long long)ctx->cus * 16);
     hipLaunchKernelGGL(k_zero, dim3(zb), dim3(256), 0, s, d_inMST, (long long)edges);
   }
-  double init_ms = 0;
   int init_launches = 0;
   int slot = 0;
-  for (int phase = 0; phase < (use_filter ? 2 : 1); phase++) {
-    if (host_sync && learned) return fail(ctx, EMST_E_ARG, "learned filter needs the device-driven loop");
+  for (int phase = 0; phase < nphases; phase++) {
     if (phase == 1) {
       const int blocks = (int)(((long long)nodes + 255) / 256);
-      hipLaunchKernelGGL(k_pick_big, dim3(1), dim3(kWave), 0, s, nodes, ctx->d_parent, &d_cnt[2]);
+      hipLaunchKernelGGL(k_pick_big, dim3(1), dim3(kWave), 0, s, nodes, ctx->d_parent, &d_cnt[2], dthr);
       hipLaunchKernelGGL(k_flatten, dim3(blocks), dim3(256), 0, s, nodes, ctx->d_parent, d_cnt, &d_cnt[2],
                          ctx->d_bigmap, dthr);
     }
     if (edges > 0) {
-      HIPCHK(hipEventRecord(ctx->ev[2 + 2 * phase], s));
-      if (phase == 0)
-        hipLaunchKernelGGL(k_init_wl<false>, dim3(init_grid), dim3(kInitWaves * kWave), init_lds, s, d_nindex, d_nlist,
-                           d_eweight, ctx->d_gsrc, nodes, edges, T, ctx->d_parent, ctx->d_bigmap, wl[0], &d_cnt[0], stage_cap, d_stats,
-                           dthr);
-      else
-        hipLaunchKernelGGL(k_init_wl<true>, dim3(init_grid), dim3(kInitWaves * kWave), init_lds, s, d_nindex, d_nlist,
-                           d_eweight, ctx->d_gsrc, nodes, edges, T, ctx->d_parent, ctx->d_bigmap, wl[0], &d_cnt[0], stage_cap, d_stats,
-                           dthr);
-      HIPCHK(hipEventRecord(ctx->ev[3 + 2 * phase], s));
+      if (!fixed) HIPCHK(hipEventRecord(ctx->ev[2 + 2 * phase], s));
+      const InitFn f = init_fn(phase == 1, sub);
+      hipLaunchKernelGGL(f, dim3(init_grid), dim3(kInitWaves * kWave), init_lds, s, d_nindex, d_nlist, d_eweight,
+                         ctx->d_gsrc, nodes, edges, T, ctx->d_parent, ctx->d_bigmap, wl[0], &d_cnt[0], stage_cap,
+                         d_stats, dthr, phase == 0 ? vgid : nullptr, phase == 0 ? gthr : nullptr);
+      if (!fixed) HIPCHK(hipEventRecord(ctx->ev[3 + 2 * phase], s));
       HIPCHK(hipGetLastError());
       init_launches++;
     }
     const int rounds_slot = phase == 0 ? ST_ROUNDS0 : ST_ROUNDS1;
     if (host_sync) {
-      // Reference schedule: read the size back after every kernel1.
+      // Reference schedule: read the worklist size back after every kernel1.
       int cur = 0;
       int size = 0;
       HIPCHK(hipMemcpyAsync(&size, &d_cnt[cur], 4, hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
+      const int tile = waves * kWave * kSelItems;
       if (size > 0) {
-        const int blocks = (int)std::min<long long>(((long long)size + kSelTile - 1) / kSelTile, round_grid);
-        hipLaunchKernelGGL(k_select<true>, dim3(blocks), dim3(kRoundWaves * kWave), 0, s, wl[cur], &d_cnt[cur],
-                           wl[cur ^ 1], &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, combine, d_stats,
-                           rounds_slot);
+        const int blocks = (int)std::min<long long>(((long long)size + tile - 1) / tile, round_grid);
+        hipLaunchKernelGGL(sel_first, dim3(blocks), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[cur ^ 1],
+                           &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot);
       }
       while (size > 0) {
         hipLaunchKernelGGL(k_join_plain, dim3((size + 255) / 256), dim3(256), 0, s, wl[cur], size, ctx->d_parent,
                            ctx->d_minv, d_inMST);
         HIPCHK(hipMemsetAsync(&d_cnt[cur ^ 1], 0, 4, s));
-        const int blocks = (int)std::min<long long>(((long long)size + kSelTile - 1) / kSelTile, round_grid);
-        hipLaunchKernelGGL(k_select<false>, dim3(blocks), dim3(kRoundWaves * kWave), 0, s, wl[cur], &d_cnt[cur],
-                           wl[cur ^ 1], &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, combine, d_stats,
-                           rounds_slot);
+        const int blocks = (int)std::min<long long>(((long long)size + tile - 1) / tile, round_grid);
+        hipLaunchKernelGGL(sel, dim3(blocks), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[cur ^ 1],
+                           &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot);
         cur ^= 1;
         HIPCHK(hipMemcpyAsync(&size, &d_cnt[cur], 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -730,24 +756,29 @@ static int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_n
     }
     // Device-driven rounds with a bounded look-ahead.
     const int lag = 2;
-    int r = 0;
-    int first_slot = slot;
-    for (;; r++) {
-      if (slot >= kHflagSlots) return fail(ctx, EMST_E_LIMIT, "round limit exceeded");
+    const int first_slot = slot;
+    for (int r = 0;; r++) {
+      if (slot >= kHflagSlots) return ctx_fail(ctx, EMST_E_LIMIT, "round limit exceeded");
       // list `cur` holds the survivors of the last select; the phase's
       // first select is the offer-only pass over the initial list
       const int cur = r & 1, nxt = cur ^ 1;
       ctx->h_flag[slot] = -1;
       if (r == 0)
-        hipLaunchKernelGGL(k_select<true>, dim3(round_grid), dim3(kRoundWaves * kWave), 0, s, wl[0], &d_cnt[0], wl[1],
-                           &d_cnt[1], ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot);
-      hipLaunchKernelGGL(k_join, dim3(round_grid), dim3(kRoundWaves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt],
-                         &d_cnt[nxt], ctx->d_parent, ctx->d_minv, d_inMST, tail_max, ctx->d_flag + slot, d_stats,
-                         rounds_slot);
-      hipLaunchKernelGGL(k_select<false>, dim3(round_grid), dim3(kRoundWaves * kWave), 0, s, wl[cur], &d_cnt[cur],
-                         wl[nxt], &d_cnt[nxt], ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot);
+        hipLaunchKernelGGL(sel_first, dim3(round_grid), dim3(waves * kWave), 0, s, wl[0], &d_cnt[0], wl[1], &d_cnt[1],
+                           ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot);
+      hipLaunchKernelGGL(join, dim3(round_grid), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt], &d_cnt[nxt],
+                         ctx->d_parent, ctx->d_minv, d_inMST, tail_max, ctx->d_flag + slot, d_stats, rounds_slot);
+      hipLaunchKernelGGL(sel, dim3(round_grid), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt], &d_cnt[nxt],
+                         ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot);
       HIPCHK(hipGetLastError());
       slot++;
+      if (fixed) {
+        if (r + 1 >= fixed->rounds[phase]) {
+          fixed->flag_slot[phase] = slot - 1;
+          break;
+        }
+        continue;
+      }
       if (r >= lag) {
         int* f = ctx->h_flag + first_slot + (r - lag);
         int v;
@@ -763,6 +794,7 @@ static int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_n
       }
     }
   }
+  if (fixed) return EMST_OK;  // captured: the caller times and reads back
   HIPCHK(hipEventRecord(ctx->ev[1], s));
   HIPCHK(hipEventSynchronize(ctx->ev[1]));
   HIPCHK(hipGetLastError());
@@ -770,6 +802,7 @@ static int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_n
     float t = 0;
     HIPCHK(hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]));
     float tph[2] = {0, 0};
+    double init_ms = 0;
     for (int ph = 0; ph < init_launches; ph++) {
       HIPCHK(hipEventElapsedTime(&tph[ph], ctx->ev[2 + 2 * ph], ctx->ev[3 + 2 * ph]));
       init_ms += tph[ph];
@@ -787,7 +820,7 @@ static int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_n
       stats->learned_q = q;
     } else {
       stats->threshold_used = T;
-      stats->filter_used = use_filter ? 1 : 0;
+      stats->filter_used = nphases == 2 ? 1 : 0;
       stats->learned_q = -1.f;
     }
     stats->ms = t;
@@ -805,6 +838,8 @@ static int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_n
   }
   return EMST_OK;
 }
+
+}  // namespace emst
 
 extern "C" {
 
@@ -836,10 +871,11 @@ int emst_create(int device, emst_ctx** out)
        hipMalloc((void**)&ctx->d_feat, kIn * 4) == hipSuccess &&
        hipMalloc((void**)&ctx->d_samples, kWeightSamples * 4) == hipSuccess;
   if (ok) {
-    ok = hipFuncSetAttribute((const void*)k_init_wl<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)ctx->lds_per_block) == hipSuccess &&
-         hipFuncSetAttribute((const void*)k_init_wl<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)ctx->lds_per_block) == hipSuccess;
+    const int dyn = (int)ctx->lds_per_block;
+    for (int h = 0; h < 2 && ok; h++)
+      for (int sub : {4, 8, 16})
+        ok = ok && hipFuncSetAttribute((const void*)init_fn(h == 1, sub), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       dyn) == hipSuccess;
   }
   if (!ok) {
     emst_destroy(ctx);
@@ -854,9 +890,10 @@ void emst_destroy(emst_ctx* ctx)
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  void* ptrs[] = {ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, ctx->d_inMST, ctx->d_parent, ctx->d_minv,
-                  ctx->d_wl[0],  ctx->d_wl[1], ctx->d_gsrc,    ctx->d_cnt,   ctx->d_stats, ctx->d_bigmap, ctx->d_mlp16, ctx->d_thr, ctx->d_q, ctx->d_feat,
-                  ctx->d_samples};
+  void* ptrs[] = {ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, ctx->d_inMST, ctx->d_parent,
+                  ctx->d_minv,   ctx->d_wl[0], ctx->d_wl[1],   ctx->d_gsrc,  ctx->d_cnt,
+                  ctx->d_stats,  ctx->d_bigmap, ctx->d_mlp16,  ctx->d_thr,   ctx->d_q,
+                  ctx->d_feat,   ctx->d_samples};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
@@ -881,13 +918,16 @@ int emst_set_config(emst_ctx* ctx, const emst_config* cfg)
   if (cfg->tile_edges) c.tile_edges = cfg->tile_edges;
   if (cfg->stage_recs) c.stage_recs = cfg->stage_recs;
   if (cfg->tail_max) c.tail_max = cfg->tail_max;
+  if (cfg->init_grid_per_cu) c.init_grid_per_cu = cfg->init_grid_per_cu;
   c.flags = cfg->flags;
-  if (c.grid_per_cu < 1 || c.grid_per_cu > 64) return fail(ctx, EMST_E_ARG, "grid_per_cu out of range");
+  if (c.grid_per_cu < 1 || c.grid_per_cu > 64 || c.init_grid_per_cu < 1 || c.init_grid_per_cu > 64)
+    return ctx_fail(ctx, EMST_E_ARG, "grid_per_cu out of range");
   if (c.stage_recs < 64 || (size_t)kInitWaves * c.stage_recs * sizeof(int4) > ctx->lds_per_block)
-    return fail(ctx, EMST_E_ARG, "stage_recs out of range");
-  if (c.tail_max > (1 << 20)) return fail(ctx, EMST_E_ARG, "tail_max out of range");
-  if (c.tile_edges != kGroupEdges) return fail(ctx, EMST_E_ARG, "tile_edges must be 256 in this build");
-  if (c.block != kRoundWaves * kWave) return fail(ctx, EMST_E_ARG, "block must be 512 in this build");
+    return ctx_fail(ctx, EMST_E_ARG, "stage_recs out of range");
+  if (c.tail_max > (1 << 20)) return ctx_fail(ctx, EMST_E_ARG, "tail_max out of range");
+  if (c.tile_edges != 256 && c.tile_edges != 512 && c.tile_edges != 1024)
+    return ctx_fail(ctx, EMST_E_ARG, "tile_edges must be 256, 512 or 1024");
+  if (c.block != 256 && c.block != 512 && c.block != 1024) return ctx_fail(ctx, EMST_E_ARG, "block must be 256, 512 or 1024");
   ctx->cfg = c;
   return EMST_OK;
 }
@@ -902,15 +942,15 @@ int emst_get_config(const emst_ctx* ctx, emst_config* cfg)
 int emst_load(emst_ctx* ctx, const emst_graph* g)
 {
   if (!ctx || !g) return EMST_E_ARG;
-  if (g->edges > 0 && !g->eweight) return fail(ctx, EMST_E_NOWEIGHT, "graph has no weights");
+  if (g->edges > 0 && !g->eweight) return ctx_fail(ctx, EMST_E_NOWEIGHT, "graph has no weights");
   int rc = emst_validate(g);
-  if (rc) return fail(ctx, rc, "CSR failed validation");
+  if (rc) return ctx_fail(ctx, rc, "CSR failed validation");
   HIPCHK(hipSetDevice(ctx->device));
   const size_t ni = (size_t)g->nodes + 1, ne = (size_t)g->edges;
   if (ni > ctx->cap_nindex || !ctx->d_nindex) {
     if (ctx->d_nindex) (void)hipFree(ctx->d_nindex);
     ctx->d_nindex = nullptr;
-    if (hipMalloc((void**)&ctx->d_nindex, ni * 4) != hipSuccess) return fail(ctx, EMST_E_NOMEM, "hipMalloc nindex");
+    if (hipMalloc((void**)&ctx->d_nindex, ni * 4) != hipSuccess) return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc nindex");
     ctx->cap_nindex = ni;
   }
   if (ne > ctx->cap_edges || !ctx->d_nlist) {
@@ -923,7 +963,7 @@ int emst_load(emst_ctx* ctx, const emst_graph* g)
     if (hipMalloc((void**)&ctx->d_nlist, want * 4) != hipSuccess ||
         hipMalloc((void**)&ctx->d_eweight, want * 4) != hipSuccess ||
         hipMalloc((void**)&ctx->d_inMST, want + 16) != hipSuccess)
-      return fail(ctx, EMST_E_NOMEM, "hipMalloc of the CSR failed");
+      return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of the CSR failed");
     ctx->cap_edges = want;
   }
   HIPCHK(hipMemcpyAsync(ctx->d_nindex, g->nindex, ni * 4, hipMemcpyHostToDevice, ctx->stream));
@@ -947,23 +987,22 @@ int emst_load(emst_ctx* ctx, const emst_graph* g)
   ctx->m = g->edges;
   ctx->m_up = up;
   ctx->loaded = true;
-  rc = ensure_buffers(ctx, g->nodes, g->edges, up);
-  return rc;
+  return ensure_buffers(ctx, g->nodes, g->edges, up, ctx->cfg.tile_edges);
 }
 
 int emst_solve(emst_ctx* ctx, int use_filter, int32_t threshold, emst_stats* stats)
 {
   if (!ctx) return EMST_E_ARG;
-  if (!ctx->loaded) return fail(ctx, EMST_E_STATE, "emst_solve before emst_load");
+  if (!ctx->loaded) return ctx_fail(ctx, EMST_E_STATE, "emst_solve before emst_load");
   HIPCHK(hipSetDevice(ctx->device));
   return run_solve(ctx, ctx->n, ctx->m, ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, use_filter, threshold,
-                   ctx->d_inMST, ctx->stream, ctx->m_up, stats);
+                   ctx->d_inMST, ctx->stream, ctx->m_up, nullptr, stats);
 }
 
 int emst_fetch(emst_ctx* ctx, uint8_t* inMST)
 {
   if (!ctx || (!inMST && ctx->m > 0)) return EMST_E_ARG;
-  if (!ctx->loaded) return fail(ctx, EMST_E_STATE, "emst_fetch before emst_load");
+  if (!ctx->loaded) return ctx_fail(ctx, EMST_E_STATE, "emst_fetch before emst_load");
   HIPCHK(hipSetDevice(ctx->device));
   if (ctx->m) HIPCHK(hipMemcpyAsync(inMST, ctx->d_inMST, (size_t)ctx->m, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -973,7 +1012,7 @@ int emst_fetch(emst_ctx* ctx, uint8_t* inMST)
 int emst_fetch_parent(emst_ctx* ctx, int32_t* parent)
 {
   if (!ctx || !parent) return EMST_E_ARG;
-  if (!ctx->d_parent || ctx->last_nodes < 1) return fail(ctx, EMST_E_STATE, "no solve has run");
+  if (!ctx->d_parent || ctx->last_nodes < 1) return ctx_fail(ctx, EMST_E_STATE, "no solve has run");
   HIPCHK(hipSetDevice(ctx->device));
   HIPCHK(hipMemcpyAsync(parent, ctx->d_parent, (size_t)ctx->last_nodes * 4, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1007,7 +1046,7 @@ int emst_set_filter_model(emst_ctx* ctx, const float* params)
   }
   std::vector<uint16_t> h(kParams);
   for (int i = 0; i < kParams; i++) {
-    if (!std::isfinite(params[i])) return fail(ctx, EMST_E_ARG, "non-finite model parameter");
+    if (!std::isfinite(params[i])) return ctx_fail(ctx, EMST_E_ARG, "non-finite model parameter");
     h[i] = emst_bf16_bits(params[i]);
   }
   HIPCHK(hipSetDevice(ctx->device));
@@ -1020,7 +1059,7 @@ int emst_learned_threshold(emst_ctx* ctx, int* use_filter, int32_t* threshold, f
                            int32_t* samples)
 {
   if (!ctx) return EMST_E_ARG;
-  if (!ctx->loaded) return fail(ctx, EMST_E_STATE, "emst_learned_threshold before emst_load");
+  if (!ctx->loaded) return ctx_fail(ctx, EMST_E_STATE, "emst_learned_threshold before emst_load");
   HIPCHK(hipSetDevice(ctx->device));
   launch_learned_threshold(ctx->stream, ctx->n, ctx->m, ctx->d_nindex, ctx->d_eweight, ctx->d_mlp16, ctx->have_model,
                            ctx->d_thr, ctx->d_q, ctx->d_feat, ctx->d_samples);
@@ -1045,7 +1084,66 @@ int emst_solve_device(emst_ctx* ctx, int32_t nodes, int32_t edges, const int32_t
   if (!ctx || !d_nindex || (edges > 0 && (!d_nlist || !d_eweight || !d_inMST))) return EMST_E_ARG;
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-  return run_solve(ctx, nodes, edges, d_nindex, d_nlist, d_eweight, use_filter, threshold, d_inMST, s, edges, stats);
+  return run_solve(ctx, nodes, edges, d_nindex, d_nlist, d_eweight, use_filter, threshold, d_inMST, s, edges, nullptr,
+                   stats);
+}
+
+// Coordinate-descent autotuner over the runtime shape knobs, timing
+// repeated solves of the loaded graph (median of `reps`).
+int emst_autotune(emst_ctx* ctx, int use_filter, int32_t threshold, int32_t reps, emst_config* best, double* best_ms)
+{
+  if (!ctx || reps < 1) return EMST_E_ARG;
+  if (!ctx->loaded) return ctx_fail(ctx, EMST_E_STATE, "emst_autotune before emst_load");
+  const emst_config saved = ctx->cfg;
+  auto measure = [&](const emst_config& c, double* ms) -> int {
+    int rc = emst_set_config(ctx, &c);
+    if (rc) return rc;
+    std::vector<double> t;
+    for (int i = 0; i < reps + 1; i++) {
+      emst_stats st;
+      rc = emst_solve(ctx, use_filter, threshold, &st);
+      if (rc) return rc;
+      if (i) t.push_back(st.ms);  // first run warms caches and buffers
+    }
+    std::sort(t.begin(), t.end());
+    *ms = t[t.size() / 2];
+    return EMST_OK;
+  };
+  emst_config cur = saved;
+  cur.flags &= ~EMST_F_HOST_SYNC;
+  double cur_ms = 0;
+  int rc = measure(cur, &cur_ms);
+  if (rc) {
+    ctx->cfg = saved;
+    return rc;
+  }
+  struct Knob {
+    int32_t emst_config::*field;
+    std::vector<int32_t> values;
+  };
+  const std::vector<Knob> knobs = {
+      {&emst_config::tile_edges, {256, 512, 1024}},   {&emst_config::stage_recs, {128, 256, 512, 1024}},
+      {&emst_config::init_grid_per_cu, {4, 8, 16}},    {&emst_config::block, {256, 512, 1024}},
+      {&emst_config::grid_per_cu, {1, 2, 4, 8}},       {&emst_config::tail_max, {-1, 1024, 4096, 16384}}};
+  for (int pass = 0; pass < 2; pass++) {
+    for (const Knob& k : knobs) {
+      for (int32_t v : k.values) {
+        if (cur.*(k.field) == v) continue;
+        emst_config c = cur;
+        c.*(k.field) = v;
+        double ms = 0;
+        if (measure(c, &ms) != EMST_OK) continue;  // e.g. LDS limits: skip
+        if (ms < cur_ms * 0.99) {
+          cur = c;
+          cur_ms = ms;
+        }
+      }
+    }
+  }
+  ctx->cfg = cur;
+  if (best) *best = cur;
+  if (best_ms) *best_ms = cur_ms;
+  return EMST_OK;
 }
 
 }  // extern "C"

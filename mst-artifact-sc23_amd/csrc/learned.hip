@@ -197,6 +197,62 @@ __global__ __launch_bounds__(kFeatThreads) void k_learned_threshold(
   }
 }
 
+// One workgroup per graph of a packed union: graph g owns vertices
+// [voff[g], voff[g+1]) and edges [eoff[g], eoff[g+1]); its nindex values are
+// union edge positions, so degrees (differences) and the weight slice at
+// eoff[g] are exactly the graph's own.  Writes the graph's threshold
+// (INT_MAX when unfiltered) and filter flag.
+__global__ __launch_bounds__(kFeatThreads) void k_learned_batch(
+    const int* __restrict__ nindex, const int* __restrict__ eweight, const long long* __restrict__ voff,
+    const long long* __restrict__ eoff, const unsigned short* __restrict__ p16, int have_model,
+    int* __restrict__ gthr, int* __restrict__ gflag)
+{
+  __shared__ float s_p[kParams];
+  __shared__ float s_feat[kIn];
+  __shared__ float s_h1[kH], s_h2[kH];
+  __shared__ int s_samples[kWeightSamples];
+  const int g = blockIdx.x;
+  const int t = threadIdx.x;
+  const int nodes = (int)(voff[g + 1] - voff[g]);
+  const int edges = (int)(eoff[g + 1] - eoff[g]);
+  const int* ni = nindex + voff[g];
+  const int* ew = eweight + eoff[g];
+  for (int i = t; i < kParams; i += blockDim.x) s_p[i] = have_model ? __uint_as_float((unsigned)p16[i] << 16) : 0.f;
+  features_block(nodes, edges, ni, ew, s_feat, s_samples);
+  if (t < kH) {
+    float a = s_p[kB1 + t];
+    for (int f = 0; f < kIn; f++) a = fmaf(s_feat[f], s_p[kW1 + f * kH + t], a);
+    s_h1[t] = a > 0.f ? a : 0.f;
+  }
+  __syncthreads();
+  if (t < kH) {
+    float a = s_p[kB2 + t];
+    for (int i = 0; i < kH; i++) a = fmaf(s_h1[i], s_p[kW2 + i * kH + t], a);
+    s_h2[t] = a > 0.f ? a : 0.f;
+  }
+  __syncthreads();
+  if (t == 0) {
+    int use_filter = 0, T = INT_MAX;
+    if (!have_model) {
+      plain_threshold_dev(nodes, edges, ew, &T, &use_filter);
+    } else if (edges > 0) {
+      float y = s_p[kB3];
+      for (int k = 0; k < kH; k++) y = fmaf(s_h2[k], s_p[kW3 + k], y);
+      if (!(y == y)) y = 0.f;
+      y = fminf(fmaxf(y, kMinLog2Q), 0.f);
+      const float q = exp2f(y);
+      if (q < kNoFilterQ) {
+        int r = (int)floorf(q * (kWeightSamples - 1));
+        r = r < 0 ? 0 : (r >= kWeightSamples ? kWeightSamples - 1 : r);
+        use_filter = 1;
+        T = s_samples[r];
+      }
+    }
+    gthr[g] = use_filter ? T : INT_MAX;
+    gflag[g] = use_filter;
+  }
+}
+
 // ---- training ---------------------------------------------------------
 // Forward for a batch: one thread per sample; stores activations.
 __global__ __launch_bounds__(256) void k_mlp_fwd(int B, const float* __restrict__ p, const float* __restrict__ X,
@@ -444,5 +500,11 @@ void launch_learned_threshold(hipStream_t s, int nodes, int edges, const int* ni
 {
   hipLaunchKernelGGL(k_learned_threshold, dim3(1), dim3(kFeatThreads), 0, s, nodes, edges, nindex, eweight, p16,
                      have_model, out, outf, feat, samples);
+}
+void launch_learned_batch(hipStream_t s, int graphs, const int* nindex, const int* eweight, const long long* voff,
+                          const long long* eoff, const unsigned short* p16, int have_model, int* gthr, int* gflag)
+{
+  hipLaunchKernelGGL(k_learned_batch, dim3(graphs), dim3(kFeatThreads), 0, s, nindex, eweight, voff, eoff, p16,
+                     have_model, gthr, gflag);
 }
 }  // namespace emst

@@ -26,4 +26,7 @@ void launch_learned_threshold(hipStream_t s, int nodes, int edges, const int* ni
                               const unsigned short* p16, int have_model, int* out, float* outf, float* feat,
                               int* samples);
 
+void launch_learned_batch(hipStream_t s, int graphs, const int* nindex, const int* eweight, const long long* voff,
+                          const long long* eoff, const unsigned short* p16, int have_model, int* gthr, int* gflag);
+
 }  // namespace emst

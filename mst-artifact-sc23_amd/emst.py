@@ -73,7 +73,11 @@ class Stats(C.Structure):
 
 class Config(C.Structure):
     _fields_ = [("block", C.c_int32), ("grid_per_cu", C.c_int32), ("tile_edges", C.c_int32),
-                ("stage_recs", C.c_int32), ("tail_max", C.c_int32), ("flags", C.c_int32)]
+                ("stage_recs", C.c_int32), ("tail_max", C.c_int32), ("flags", C.c_int32),
+                ("init_grid_per_cu", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 def _sig(name, res, *args):
@@ -125,6 +129,25 @@ _mlp_grad = _sig("emst_mlp_grad_batch", C.c_int, C.c_int, C.c_int32, _pf32, _pf3
 _mlp_train = _sig("emst_mlp_train", C.c_int, C.c_int, C.c_int32, _pf32, _pf32, C.c_int32, C.c_float, C.c_uint64,
                   _pf32, C.POINTER(C.c_float))
 _bf16_bits = _sig("emst_bf16_bits", C.c_uint16, C.c_float)
+class BatchStats(C.Structure):
+    _fields_ = [("ms", C.c_double), ("chunks", C.c_int32), ("graphs", C.c_int32), ("nodes", C.c_int64),
+                ("edges", C.c_int64), ("light", C.c_int64), ("heavy", C.c_int64), ("light_rounds", C.c_int32),
+                ("heavy_rounds", C.c_int32), ("replays", C.c_int32), ("filtered_graphs", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_batch_create = _sig("emst_batch_create", C.c_int, C.c_void_p, C.c_int32, C.POINTER(_Graph), C.c_int64,
+                     C.POINTER(C.c_void_p))
+_batch_destroy = _sig("emst_batch_destroy", None, C.c_void_p)
+_batch_solve = _sig("emst_batch_solve", C.c_int, C.c_void_p, C.c_int, _p32, C.c_int, C.POINTER(BatchStats))
+_batch_fetch = _sig("emst_batch_fetch", C.c_int, C.c_void_p, C.c_int32, _pu8)
+_batch_thresholds = _sig("emst_batch_thresholds", C.c_int, C.c_void_p, _p32, _p32)
+_batch_info = _sig("emst_batch_info", C.c_int, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                   C.POINTER(C.c_int64))
+_autotune = _sig("emst_autotune", C.c_int, C.c_void_p, C.c_int, C.c_int32, C.c_int32, C.POINTER(Config),
+                 C.POINTER(C.c_double))
 
 
 def _check(rc, msg=""):
@@ -347,6 +370,12 @@ class Solver:
         return st
 
 
+    def autotune(self, use_filter, threshold, reps=3):
+        """Runs the autotuner on the loaded graph; returns (config dict, ms)."""
+        c, ms = Config(), C.c_double()
+        self._err(_autotune(self._h, int(use_filter), int(threshold), int(reps), C.byref(c), C.byref(ms)))
+        return c.as_dict(), ms.value
+
     def set_filter_model(self, params):
         """Loads the threshold network (MLP_PARAMS fp32 values); None removes it."""
         if params is None:
@@ -414,3 +443,52 @@ def init_params(seed=0):
     p[2112:6208] = rng.normal(0, np.sqrt(2 / 64), 4096)
     p[6272:6336] = rng.normal(0, np.sqrt(1 / 64), 64)
     return p
+
+
+class Batch:
+    """Many graphs packed into device arenas and solved together."""
+
+    def __init__(self, solver, graphs, max_arena_edges=0):
+        self.solver = solver
+        self.graphs = list(graphs)
+        self._views = (_Graph * len(self.graphs))(*[g.view() for g in self.graphs])
+        h = C.c_void_p()
+        solver._err(_batch_create(solver._h, len(self.graphs), self._views, int(max_arena_edges), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            _batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        c, n, m = C.c_int32(), C.c_int64(), C.c_int64()
+        self.solver._err(_batch_info(self._h, C.byref(c), C.byref(n), C.byref(m)))
+        return c.value, n.value, m.value
+
+    def solve(self, use_filter=FILTER_PLAIN, thresholds=None, use_graph=True):
+        st = BatchStats()
+        tp = None
+        if thresholds is not None:
+            self._thr = np.ascontiguousarray(thresholds, np.int32)
+            tp = self._thr.ctypes.data_as(_p32)
+        self.solver._err(_batch_solve(self._h, int(use_filter), tp, int(bool(use_graph)), C.byref(st)))
+        return st
+
+    def fetch(self, g):
+        m = self.graphs[g].edges
+        out = np.zeros(max(m, 1), np.uint8)
+        self.solver._err(_batch_fetch(self._h, int(g), out.ctypes.data_as(_pu8)))
+        return out[:m]
+
+    def thresholds(self):
+        t = np.zeros(len(self.graphs), np.int32)
+        f = np.zeros(len(self.graphs), np.int32)
+        self.solver._err(_batch_thresholds(self._h, t.ctypes.data_as(_p32), f.ctypes.data_as(_p32)))
+        return t, f

@@ -194,3 +194,21 @@ def test_solve_device_pointers(solver):
         for p in bufs:
             hip.hipFree(p)
     assert np.array_equal(got, O.kruskal(g))
+
+
+def test_shapes_and_autotune_exact(solver):
+    g = emst.gen_rmat(15, 16, 0.57, 0.19, 0.19, 3, 1, 1 << 12)
+    uf, T = emst.plain_threshold(g)
+    ref = O.kruskal(g)
+    for kw in (dict(block=256), dict(block=1024), dict(tile_edges=256), dict(tile_edges=1024),
+               dict(stage_recs=128, init_grid_per_cu=2), dict(block=1024, tile_edges=1024, stage_recs=1024)):
+        solver.set_config(**kw)
+        got, _ = solver.gpu_mst(g, uf, T)
+        assert np.array_equal(got, ref), kw
+    solver.set_config()
+    solver.load(g)
+    cfg, ms = solver.autotune(uf, T, reps=1)
+    assert ms > 0 and cfg["block"] in (256, 512, 1024)
+    solver.solve(uf, T)
+    assert np.array_equal(solver.fetch(), ref)
+    solver.set_config()
