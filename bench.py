@@ -1,33 +1,48 @@
 """Benchmark of the MI355X ECL-MST solver.
 
-Workload (one GPU): RMAT scale 24, edge factor 16, (a,b,c,d) = (0.57, 0.19,
-0.19, 0.05), seed 2, symmetrized, self-loops and duplicates removed, weights
-uniform int32 in [1, 2^24) mirrored on both directions.  A step is one
-device-resident solve (the reference's timed region, ECL_MST.cu:301-345)
-with the plain threshold (ECL_MST.cu:432-443).  The metric is directed CSR
-edges per second (generate_compare_tables.py:55-59 uses Medges/s; this
-reports Gedges/s).
+Default workload (one GPU): RMAT scale 24, edge factor 16, (a,b,c,d) =
+(0.57, 0.19, 0.19, 0.05), seed 2, symmetrized, self-loops and duplicates
+removed, weights uniform int32 in [1, 2^24) mirrored on both directions.
+A step is one device-resident solve (the reference's timed region,
+ECL_MST.cu:301-345).  Two paths are measured, both exact:
+  learned -- light-edge threshold predicted on the device by the trained
+             network (mst-artifact-sc23_amd/filter_mlp.f32); the headline
+  plain   -- the reference's threshold rule (ECL_MST.cu:432-443)
+The metric is directed CSR edges per second (generate_compare_tables.py:
+55-59 reports Medges/s; this reports Gedges/s).
+
+--workload batch runs the batched configuration instead (4,096 graphs,
+n log-uniform in [1e3, 1e6], generator uniform over grid / geometric deg 6 /
+RMAT ef 8 / Erdos-Renyi deg 10, weights in [1, 2^20), seed 4) and reports
+graphs/s with p50/p99 per-graph latency against the per-graph loop.
 
 Prints one JSON line.  --gpus above 1 is refused: this build measures one
 GPU.
 """
 import argparse
 import json
+import math
 import os
 import statistics
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
-sys.path.insert(0, os.path.join(ROOT, "mst-artifact-sc23_amd"))
+PKG = os.path.join(ROOT, "mst-artifact-sc23_amd")
+sys.path.insert(0, PKG)
+PEAK_HBM_GBS = 8000.0
 
 WORKLOADS = {
     "rmat24": dict(kind="rmat", scale=24, ef=16, seed=2, wlo=1, whi=1 << 24,
                    desc="RMAT scale 24, edge factor 16, (0.57,0.19,0.19,0.05), seed 2, w in [1,2^24)"),
+    "rmat26": dict(kind="rmat", scale=26, ef=16, seed=5, wlo=1, whi=65,
+                   desc="RMAT scale 26, edge factor 16, (0.57,0.19,0.19,0.05), seed 5, w in [1,64]"),
     "grid512": dict(kind="grid", rows=512, cols=512, seed=1, wlo=1, whi=1 << 20,
                     desc="512x512 4-neighbour grid, seed 1, w in [1,2^20)"),
     "rgg24": dict(kind="rgg", n=1 << 24, deg=6.0, seed=3, wmax=1 << 24,
-                  desc="random geometric graph, 2^24 points, mean degree 6, seed 3"),
+                  desc="random geometric graph, 2^24 points, mean degree 6, seed 3, w in [1,2^24)"),
 }
 
 
@@ -39,19 +54,243 @@ def make_graph(emst, w, scale=None):
     return emst.gen_rgg(w["n"], w["deg"], w["seed"], w["wmax"])
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(emst, w, sample_scale):
-    """Oracle serial Kruskal (restating ECL_MST.cu:82-124) on a bounded
-    sample: the same generator at a smaller scale, timed on one host core."""
+    """The oracle's serial Kruskal (restating ECL_MST.cu:82-124) on a bounded
+    sample, on one host core."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     g = make_graph(emst, w, scale=sample_scale) if w["kind"] == "rmat" else make_graph(emst, w)
     t0 = time.perf_counter()
     oracle_lib.kruskal(g)
     dt = time.perf_counter() - t0
+    what = (f"RMAT scale {sample_scale} with the same generator parameters" if w["kind"] == "rmat"
+            else "the full workload")
     return {"value": g.edges / dt / 1e9, "unit": "Gedges/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle serial Kruskal on RMAT scale {sample_scale} (same generator and parameters, "
-                       f"m={g.edges} directed edges), {dt:.2f} s" if w["kind"] == "rmat"
-                       else f"oracle serial Kruskal on the full workload, m={g.edges}, {dt:.2f} s")}
+            "sample": f"oracle serial Kruskal on {what}: m={g.edges} directed edges in {dt:.2f} s",
+            "host_cpus": os.cpu_count()}
+
+
+def timed_steps(s, mode, thr, steps, warmup):
+    for _ in range(warmup):
+        s.solve(mode, thr)
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stats.append(s.solve(mode, thr))  # each solve ends with a stream synchronize
+    wall = time.perf_counter() - t0
+    return wall * 1e3 / steps, stats
+
+
+def read_traffic(name):
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f)
+
+
+def run_single(a, emst):
+    w = WORKLOADS[a.workload]
+    t0 = time.perf_counter()
+    g = make_graph(emst, w, a.scale)
+    log(f"generated {a.workload}: n={g.nodes} m={g.edges} in {time.perf_counter()-t0:.1f}s")
+    s = emst.Solver(0)
+    s.load(g)
+    model = os.path.join(PKG, "filter_mlp.f32")
+    s.set_filter_model(np.fromfile(model, np.float32))
+    uf, T = emst.plain_threshold(g)
+    plain_mode = emst.FILTER_PLAIN if uf else emst.FILTER_OFF
+    if a.autotune:
+        cfg, ms = s.autotune(emst.FILTER_LEARNED, 0, reps=3)
+        log(f"autotuned: {cfg} {ms:.3f} ms")
+    ms_learned, st_l = timed_steps(s, emst.FILTER_LEARNED, 0, a.steps, a.warmup)
+    learned_inmst = s.fetch()
+    log(f"learned: {ms_learned:.3f} ms/step")
+    ms_plain, st_p = timed_steps(s, plain_mode, T, a.steps, a.warmup)
+    plain_inmst = s.fetch()
+    log(f"plain: {ms_plain:.3f} ms/step")
+    exact = None
+    if not a.no_verify:
+        ref = emst.cpu_mst(g)
+        d1, cw, gw1 = emst.verify(g, ref, learned_inmst)
+        d2, _, gw2 = emst.verify(g, ref, plain_inmst)
+        exact = (not d1) and (not d2) and cw == gw1 == gw2
+        log(f"verified against the host Kruskal: exact={exact} weight={cw}")
+    m_up = int(g.upper_mask().sum())
+    stl = st_l[-1]
+    n, m = g.nodes, g.edges
+    # Algorithmic bytes of the initialisation passes (the byte model of the
+    # reference schedule: nindex 4(n+1), nlist 4m, upper-direction weights
+    # 4*m_up, records 16 B; the heavy pass adds one root read per vertex and
+    # per heavy upper-direction edge).
+    heavy_up = m_up - stl.light
+    heavy_bytes = 4 * (n + 1) + 4 * m + 4 * m_up + 4 * n + 4 * heavy_up + 16 * stl.heavy
+    light_bytes = 4 * (n + 1) + 4 * m + 4 * m_up + 16 * stl.light
+    heavy_ms = statistics.mean(x.init_heavy_ms for x in st_l)
+    light_ms = statistics.mean(x.init_light_ms for x in st_l)
+    tr = read_traffic(f"traffic_{a.workload}.json") if a.scale is None else None
+    dom = {"kernel": "k_init_wl<heavy> (heavy worklist initialisation, largest launch)", "bound": "hbm",
+           "achieved": heavy_bytes / (heavy_ms * 1e-3) / 1e9 if stl.filter_used else None,
+           "peak": PEAK_HBM_GBS, "unit": "GB/s", "traffic": tr.get("k_init_wl<heavy>") if tr else None,
+           "alg_bytes_per_launch": heavy_bytes, "avg_launch_ms": heavy_ms}
+    if dom["achieved"] is None:  # unfiltered: the light pass is the only initialisation
+        dom = {"kernel": "k_init_wl<light>", "bound": "hbm", "achieved": light_bytes / (light_ms * 1e-3) / 1e9,
+               "peak": PEAK_HBM_GBS, "unit": "GB/s", "traffic": tr.get("k_init_wl<light>") if tr else None,
+               "alg_bytes_per_launch": light_bytes, "avg_launch_ms": light_ms}
+    dom["frac"] = dom["achieved"] / PEAK_HBM_GBS
+    light_roof = {"kernel": "k_init_wl<light>", "achieved": light_bytes / (light_ms * 1e-3) / 1e9,
+                  "frac": light_bytes / (light_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, "alg_bytes_per_launch": light_bytes,
+                  "avg_launch_ms": light_ms, "traffic": tr.get("k_init_wl<light>") if tr else None}
+    out = {
+        "metric": "MST Gedges/s at exact weight (directed CSR edges / device-resident solve), learned filter",
+        "value": m / (ms_learned * 1e-3) / 1e9,
+        "unit": "Gedges/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_learned,
+        "median_device_ms": statistics.median(x.ms for x in st_l),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32 (weights, ids) / u64 (keys)",
+        "data": "synthetic",
+        "exact": exact,
+        "plain": {"value": m / (ms_plain * 1e-3) / 1e9, "ms_per_step": ms_plain,
+                  "median_device_ms": statistics.median(x.ms for x in st_p), "threshold": T, "filter": bool(uf),
+                  "light": st_p[-1].light, "heavy": st_p[-1].heavy,
+                  "rounds": [st_p[-1].light_rounds, st_p[-1].heavy_rounds]},
+        "learned_vs_plain": ms_plain / ms_learned,
+        "config": {"workload": w["desc"] if a.scale is None else w["desc"].replace(str(w.get("scale")), str(a.scale), 1),
+                   "nodes": n, "edges": m, "learned_q": stl.learned_q, "threshold": stl.threshold_used,
+                   "filter": bool(stl.filter_used), "light": stl.light, "heavy": stl.heavy,
+                   "rounds": [stl.light_rounds, stl.heavy_rounds], "tail_rounds": stl.tail_rounds,
+                   "solver_config": s.config().as_dict()},
+        "roofline": dom,
+        "roofline_light_init": light_roof,
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(emst, w, a.cpu_sample_scale)
+    s.close()
+    return out
+
+
+def batch_graphs(emst, count, seed, nmin, nmax):
+    """The batched workload: count graphs, n log-uniform in [nmin, nmax],
+    generator uniform over {grid, geometric deg 6, RMAT ef 8, ER deg 10}."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(math.exp(rng.uniform(math.log(nmin), math.log(nmax))))
+        kind = int(rng.integers(0, 4))
+        gs = int(rng.integers(1, 1 << 31))
+        if kind == 0:
+            r = max(1, int(math.sqrt(n)))
+            out.append(("grid", emst.gen_grid(r, max(1, n // r), gs, 1, 1 << 20)))
+        elif kind == 1:
+            out.append(("rgg", emst.gen_rgg(n, 6.0, gs, 1 << 20)))
+        elif kind == 2:
+            out.append(("rmat", emst.gen_rmat(max(1, round(math.log2(n))), 8, 0.57, 0.19, 0.19, gs, 1, 1 << 20)))
+        else:
+            out.append(("er", emst.gen_er(n, 10.0, gs, 1, 1 << 20)))
+        if i % 256 == 0:
+            log(f"generated {i}/{count}")
+    return out
+
+
+def run_batch(a, emst):
+    t0 = time.perf_counter()
+    kg = batch_graphs(emst, a.batch_graphs, 4, a.batch_nmin, a.batch_nmax)
+    graphs = [g for _, g in kg]
+    tot_n = sum(g.nodes for g in graphs)
+    tot_m = sum(g.edges for g in graphs)
+    log(f"generated {len(graphs)} graphs, {tot_n} vertices, {tot_m} edges in {time.perf_counter()-t0:.0f}s")
+    s = emst.Solver(0)
+    s.set_filter_model(np.fromfile(os.path.join(PKG, "filter_mlp.f32"), np.float32))
+    b = emst.Batch(s, graphs)
+    for _ in range(a.warmup):
+        b.solve(emst.FILTER_LEARNED, use_graph=True)
+    bt = []
+    for _ in range(a.steps):
+        t1 = time.perf_counter()
+        st = b.solve(emst.FILTER_LEARNED, use_graph=True)
+        bt.append(time.perf_counter() - t1)
+    batch_s = statistics.median(bt)
+    plain_b = []
+    for _ in range(max(1, a.steps // 3)):
+        t1 = time.perf_counter()
+        b.solve(emst.FILTER_PLAIN, use_graph=True)
+        plain_b.append(time.perf_counter() - t1)
+    # per-graph loop (plain rule, one solve per graph, data resident)
+    sample = list(range(0, len(graphs), max(1, len(graphs) // a.loop_sample)))
+    lat = []
+    exact = True
+    for i in sample:
+        g = graphs[i]
+        s.load(g)
+        uf, T = emst.plain_threshold(g)
+        s.solve(emst.FILTER_PLAIN if uf else emst.FILTER_OFF, T)
+        t1 = time.perf_counter()
+        s.solve(emst.FILTER_PLAIN if uf else emst.FILTER_OFF, T)
+        lat.append(time.perf_counter() - t1)
+        if i in sample[:a.verify_graphs]:
+            ref = emst.cpu_mst(g)
+            exact = exact and bool(np.array_equal(b.fetch(i), ref))
+    loop_total_s = statistics.mean(lat) * len(graphs)
+    out = {
+        "metric": "batched graphs/s (exact MSF per graph), learned filter, one batch per step",
+        "value": len(graphs) / batch_s,
+        "unit": "graphs/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": batch_s * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32 (weights, ids) / u64 (keys)", "data": "synthetic",
+        "exact_sampled": exact,
+        "config": {"workload": f"{len(graphs)} graphs, n log-uniform [{a.batch_nmin}, {a.batch_nmax}], "
+                               "grid / geometric deg 6 / RMAT ef 8 / ER deg 10, w in [1,2^20), seed 4",
+                   "nodes": tot_n, "edges": tot_m, "arenas": st.chunks, "replays": st.replays,
+                   "filtered_graphs": st.filtered_graphs},
+        "batch_latency_ms": {"p50": batch_s * 1e3, "p99": statistics.quantiles(bt, n=100)[98] * 1e3
+                             if len(bt) >= 2 else batch_s * 1e3,
+                             "note": "every graph of a batch completes with the batch"},
+        "batch_plain_ms": statistics.median(plain_b) * 1e3,
+        "per_graph_loop": {"graphs_per_s": len(graphs) / loop_total_s, "p50_ms": float(np.percentile(lat, 50)) * 1e3,
+                           "p99_ms": float(np.percentile(lat, 99)) * 1e3, "sampled_graphs": len(sample),
+                           "total_s_estimated": loop_total_s},
+        "speedup_vs_loop": loop_total_s / batch_s,
+    }
+    b.close()
+    s.close()
+    return out
+
+
+def run_copy(a):
+    """Calibration for HBM traffic counters: the HIP runtime's device-to-
+    device copy of a known size (read 1 GiB, write 1 GiB per launch).  Run
+    under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE to obtain the factor
+    between counter readings and bytes moved."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    nbytes = 1 << 30
+    x, y = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(x), nbytes) == 0 and hip.hipMalloc(C.byref(y), nbytes) == 0
+    assert hip.hipMemset(x, 1, nbytes) == 0
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        assert hip.hipMemcpy(y, x, nbytes, 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    dt = (time.perf_counter() - t0) / a.steps
+    hip.hipFree(x)
+    hip.hipFree(y)
+    return {"metric": "device copy GB/s (traffic calibration)", "value": 2 * nbytes / dt / 1e9, "unit": "GB/s",
+            "bytes_read_per_copy": nbytes, "bytes_written_per_copy": nbytes, "steps": a.steps}
 
 
 def main():
@@ -59,75 +298,27 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=9)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="rmat24", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="rmat24", choices=sorted(WORKLOADS) + ["batch", "copy"])
     ap.add_argument("--scale", type=int, default=None, help="override the RMAT scale (testing only)")
+    ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-scale", type=int, default=22)
+    ap.add_argument("--cpu-sample-scale", type=int, default=23)
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--batch-graphs", type=int, default=4096)
+    ap.add_argument("--batch-nmin", type=int, default=1000)
+    ap.add_argument("--batch-nmax", type=int, default=1000000)
+    ap.add_argument("--loop-sample", type=int, default=256)
+    ap.add_argument("--verify-graphs", type=int, default=32)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported by this build "
-              "(the path is single-GPU; see DESIGN.md for how it would shard)", file=sys.stderr)
+              "(the path is single-GPU; DESIGN.md describes how it would shard)", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "copy":
+        print(json.dumps(run_copy(a)))
+        return
     import emst
-    w = WORKLOADS[a.workload]
-    t0 = time.perf_counter()
-    g = make_graph(emst, w, a.scale)
-    print(f"[bench] generated {a.workload}: n={g.nodes} m={g.edges} in {time.perf_counter()-t0:.1f}s",
-          file=sys.stderr, flush=True)
-    use_filter, T = emst.plain_threshold(g)
-    s = emst.Solver(0)
-    s.load(g)
-    m_up = int(g.upper_mask().sum())
-    for _ in range(a.warmup):
-        s.solve(use_filter, T)
-    print("[bench] warmup done", file=sys.stderr, flush=True)
-    stats = []
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        stats.append(s.solve(use_filter, T))  # each solve ends with a stream synchronize
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / a.steps
-    dev_ms = [st.ms for st in stats]
-    st = stats[-1]
-    exact = None
-    if not a.no_verify:
-        got = s.fetch()
-        ref = emst.cpu_mst(g)
-        differs, cw, gw = emst.verify(g, ref, got)
-        exact = (not differs) and cw == gw
-    # Dominant kernel: the light worklist initialisation (CSR stream).
-    # Algorithmic bytes per launch: nindex 4(n+1) + nlist 4m + upper weights
-    # 4*m_up + records 16*L.
-    light_ms = statistics.mean(x.init_light_ms for x in stats)
-    alg_bytes = 4 * (g.nodes + 1) + 4 * g.edges + 4 * m_up + 16 * st.light
-    achieved = alg_bytes / (light_ms * 1e-3) / 1e9
-    out = {
-        "metric": "MST Gedges/s (directed CSR edges / device-resident solve time), exact weight",
-        "value": g.edges / (ms_per_step * 1e-3) / 1e9,
-        "unit": "Gedges/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": ms_per_step,
-        "median_device_ms": statistics.median(dev_ms),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int32/u64",
-        "data": "synthetic",
-        "exact": exact,
-        "config": {"workload": w["desc"] if a.scale is None else w["desc"].replace("24", str(a.scale), 1),
-                   "nodes": g.nodes, "edges": g.edges, "filter": bool(use_filter), "threshold": T,
-                   "light": st.light, "heavy": st.heavy, "light_rounds": st.light_rounds,
-                   "heavy_rounds": st.heavy_rounds, "tail_rounds": st.tail_rounds},
-        "roofline": {"kernel": "k_init_wl<light>", "bound": "hbm", "achieved": achieved, "peak": 8000.0,
-                     "unit": "GB/s", "frac": achieved / 8000.0, "traffic": None,
-                     "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": light_ms},
-    }
-    if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(emst, w, a.cpu_sample_scale)
-    s.close()
+    out = run_batch(a, emst) if a.workload == "batch" else run_single(a, emst)
     print(json.dumps(out))
 
 

@@ -125,7 +125,7 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
     const int* __restrict__ gsrc, int nodes, int edges, int T, const int* __restrict__ label,
     const unsigned* __restrict__ bigmap, int4* __restrict__ wl, int* __restrict__ cnt, int stage_cap,
     long long* __restrict__ stats, const int* __restrict__ dthr, const int* __restrict__ vgid,
-    const int* __restrict__ gthr)
+    const int* __restrict__ gthr, const int* __restrict__ bigroot_p)
 {
   constexpr int TE = SUB * kWave;
   extern __shared__ int4 lds_stage[];
@@ -133,6 +133,7 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
     if (HEAVY && dthr[1] == 0) return;  // no heavy phase when unfiltered
     T = dthr[0];
   }
+  const int bigroot = HEAVY ? *bigroot_p : -1;
   const int lane = lane_id();
   const int wid = threadIdx.x / kWave;
   Stage st{lds_stage + wid * stage_cap, stage_cap, 0};
@@ -183,30 +184,33 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
         stage_push(st, keep, make_int4(src[u], nl[u], w[u], j), wl, cnt);
       }
     } else {
-      bool both[SUB];
+      // bit 0: upper direction, bit 1: source marked, bit 2: target marked
+      int flag[SUB];
 #pragma unroll
       for (int u = 0; u < SUB; u++) {
-        both[u] = true;
+        flag[u] = 0;
         if (nl[u] > src[u]) {
-          const unsigned bu = bigmap[(unsigned)nl[u] >> 5] >> (nl[u] & 31);
-          const unsigned bv = bigmap[(unsigned)src[u] >> 5] >> (src[u] & 31);
-          both[u] = (bu & bv & 1u) != 0;
+          const unsigned bu = (bigmap[(unsigned)nl[u] >> 5] >> (nl[u] & 31)) & 1u;
+          const unsigned bv = (bigmap[(unsigned)src[u] >> 5] >> (src[u] & 31)) & 1u;
+          flag[u] = 1 | (int)(bv << 1) | (int)(bu << 2);
         }
       }
+      // a marked endpoint's root is the marked component's root, so only
+      // unmarked endpoints need a root lookup
       int a[SUB], b[SUB];
 #pragma unroll
       for (int u = 0; u < SUB; u++) {
-        a[u] = 0;
-        b[u] = 0;
-        if (!both[u]) {
-          a[u] = label[src[u]];
-          b[u] = label[nl[u]];
+        a[u] = bigroot;
+        b[u] = bigroot;
+        if ((flag[u] & 1) && (flag[u] & 6) != 6) {
+          if (!(flag[u] & 2)) a[u] = label[src[u]];
+          if (!(flag[u] & 4)) b[u] = label[nl[u]];
         }
       }
 #pragma unroll
       for (int u = 0; u < SUB; u++) {
         const long long p = gbeg + u * kWave + lane;
-        const bool keep = !both[u] && a[u] != b[u];
+        const bool keep = (flag[u] & 1) && a[u] != b[u];
         const int wt = keep ? eweight[p] : 0;
         emitted += keep;
         stage_push(st, keep, make_int4(a[u], b[u], wt, (int)p), wl, cnt);
@@ -525,7 +529,8 @@ __global__ __launch_bounds__(256) void k_join_plain(const int4* __restrict__ wl,
 
 // ---- runtime selection of template instances -------------------------------
 typedef void (*InitFn)(const int*, const int*, const int*, const int*, int, int, int, const int*,
-                       const unsigned*, int4*, int*, int, long long*, const int*, const int*, const int*);
+                       const unsigned*, int4*, int*, int, long long*, const int*, const int*, const int*,
+                       const int*);
 typedef void (*SelFn)(const int4*, const int*, int4*, int*, const int*, u64*, int, int, long long*, int);
 typedef void (*JoinFn)(int4*, int*, int4*, int*, int*, u64*, uint8_t*, int, int*, long long*, int);
 
@@ -722,7 +727,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       const InitFn f = init_fn(phase == 1, sub);
       hipLaunchKernelGGL(f, dim3(init_grid), dim3(kInitWaves * kWave), init_lds, s, d_nindex, d_nlist, d_eweight,
                          ctx->d_gsrc, nodes, edges, T, ctx->d_parent, ctx->d_bigmap, wl[0], &d_cnt[0], stage_cap,
-                         d_stats, dthr, phase == 0 ? vgid : nullptr, phase == 0 ? gthr : nullptr);
+                         d_stats, dthr, phase == 0 ? vgid : nullptr, phase == 0 ? gthr : nullptr, &d_cnt[2]);
       if (!fixed) HIPCHK(hipEventRecord(ctx->ev[3 + 2 * phase], s));
       HIPCHK(hipGetLastError());
       init_launches++;
