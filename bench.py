@@ -58,20 +58,34 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(emst, w, sample_scale):
-    """The oracle's serial Kruskal (restating ECL_MST.cu:82-124) on a bounded
-    sample, on one host core."""
+def cpu_baseline(emst, w, sample_scale, g_full):
+    """The oracle's serial Kruskal (restating ECL_MST.cu:82-124), timed on
+    one host core, plus the two parallel CPU restatements (OpenMP Boruvka in
+    the structure of UMinho_BOTH/src/BoruvkaUMinho_OMP.cpp, and
+    Filter-Kruskal, the method of PBBS parallelFilterKruskal) on all host
+    cores, on the same input.  sample_scale 0 times the full workload."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
-    g = make_graph(emst, w, scale=sample_scale) if w["kind"] == "rmat" else make_graph(emst, w)
+    if sample_scale and w["kind"] == "rmat" and sample_scale != w["scale"]:
+        g = make_graph(emst, w, scale=sample_scale)
+        what = f"RMAT scale {sample_scale} with the same generator parameters"
+    else:
+        g = g_full
+        what = "the full workload"
     t0 = time.perf_counter()
-    oracle_lib.kruskal(g)
+    ref = oracle_lib.kruskal(g)
     dt = time.perf_counter() - t0
-    what = (f"RMAT scale {sample_scale} with the same generator parameters" if w["kind"] == "rmat"
-            else "the full workload")
+    cores = os.cpu_count()
+    par = {}
+    for name, fn in (("openmp_boruvka", oracle_lib.boruvka_omp), ("openmp_filter_kruskal", oracle_lib.filter_kruskal_omp)):
+        t1 = time.perf_counter()
+        got = fn(g, cores)
+        d = time.perf_counter() - t1
+        par[name] = {"value": g.edges / d / 1e9, "unit": "Gedges/s", "cores": cores, "seconds": d,
+                     "exact": bool(np.array_equal(got, ref))}
     return {"value": g.edges / dt / 1e9, "unit": "Gedges/s", "cores": 1, "kind": "port",
             "sample": f"oracle serial Kruskal on {what}: m={g.edges} directed edges in {dt:.2f} s",
-            "host_cpus": os.cpu_count()}
+            "parallel_restatements": par}
 
 
 def timed_steps(s, mode, thr, steps, warmup):
@@ -174,7 +188,7 @@ def run_single(a, emst):
         "roofline_light_init": light_roof,
     }
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(emst, w, a.cpu_sample_scale)
+        out["cpu_baseline"] = cpu_baseline(emst, w, a.cpu_sample_scale, g)
     s.close()
     return out
 
@@ -302,7 +316,7 @@ def main():
     ap.add_argument("--scale", type=int, default=None, help="override the RMAT scale (testing only)")
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-scale", type=int, default=23)
+    ap.add_argument("--cpu-sample-scale", type=int, default=0, help="RMAT scale of the CPU sample (0: full workload)")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--batch-graphs", type=int, default=4096)
     ap.add_argument("--batch-nmin", type=int, default=1000)

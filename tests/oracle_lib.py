@@ -121,6 +121,24 @@ def schedule(g, use_filter, threshold):
     return out[:len(nl)], st
 
 
+lib.orc_boruvka_omp.argtypes = [C.POINTER(_G), _pu8, C.c_int]
+lib.orc_filter_kruskal_omp.argtypes = [C.POINTER(_G), _pu8, C.c_int]
+
+
+def boruvka_omp(g, threads=0):
+    ni, nl, ew = _arrs(g)
+    out = np.zeros(max(len(nl), 1), np.uint8)
+    assert lib.orc_boruvka_omp(C.byref(_view(ni, nl, ew)), out.ctypes.data_as(_pu8), threads) == 0
+    return out[:len(nl)]
+
+
+def filter_kruskal_omp(g, threads=0):
+    ni, nl, ew = _arrs(g)
+    out = np.zeros(max(len(nl), 1), np.uint8)
+    assert lib.orc_filter_kruskal_omp(C.byref(_view(ni, nl, ew)), out.ctypes.data_as(_pu8), threads) == 0
+    return out[:len(nl)]
+
+
 def verify(g, cpu, gpu):
     ni, nl, ew = _arrs(g)
     cpu = np.ascontiguousarray(cpu, np.uint8)

@@ -250,3 +250,17 @@ def test_oracle_reader_matches_reference_reader(tmp_path):
         assert h1 == fnv(ni) and h2 == fnv(nl), name
         if ew is not None:
             assert h3 == fnv(ew), name
+
+
+def test_parallel_cpu_baselines_match_kruskal():
+    """The OpenMP Boruvka and Filter-Kruskal baselines are independent MSF
+    algorithms; both must reproduce the oracle's Kruskal exactly."""
+    import emst
+    rng = np.random.default_rng(21)
+    graphs = [G(*random_graph(rng, int(rng.integers(1, 200)), int(rng.integers(0, 800)), -3, 6)) for _ in range(40)]
+    graphs += [emst.gen_rmat(14, 16, seed=2), emst.gen_grid(60, 70, 1), emst.gen_rgg(30000, 6.0, 3),
+               emst.gen_rmat(13, 16, seed=5, wlo=1, whi=65)]
+    for g in graphs:
+        k = O.kruskal(g)
+        assert np.array_equal(O.boruvka_omp(g, 4), k)
+        assert np.array_equal(O.filter_kruskal_omp(g, 4), k)
