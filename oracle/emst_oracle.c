@@ -362,3 +362,43 @@ int orc_verify(const orc_graph* g, const uint8_t* cpu, const uint8_t* gpu,
   if (only_gpu) *only_gpu = og;
   return (cw != gw || oc != 0 || og != 0) ? 1 : 0;
 }
+
+/* Size-independent checks of a candidate forest (no reference counterpart;
+ * pinned by the unit tests against Kruskal): union-find over the marked
+ * edges detects a cycle (not a forest) or a marked non-upper edge; a second
+ * union-find over all upper-direction edges counts the components c of the
+ * graph.  A spanning forest has exactly n - c edges. */
+int orc_forest_check(const orc_graph* g, const uint8_t* inMST, int64_t* marked, int64_t* components,
+                     int* is_forest)
+{
+  const int32_t n = g->nodes;
+  int32_t* p = (int32_t*)malloc((size_t)n * 4);
+  if (!p) return ORC_E_ALLOC;
+  for (int32_t v = 0; v < n; v++) p[v] = v;
+  int64_t mk = 0;
+  int ok = 1;
+  for (int32_t v = 0; v < n; v++)
+    for (int32_t j = g->nindex[v]; j < g->nindex[v + 1]; j++) {
+      if (!inMST[j]) continue;
+      mk++;
+      const int32_t u = g->nlist[j];
+      if (u <= v) { ok = 0; continue; }
+      const int32_t a = uf_find(v, p), b = uf_find(u, p);
+      if (a == b) ok = 0; else uf_union_min(a, b, p);
+    }
+  for (int32_t v = 0; v < n; v++) p[v] = v;
+  int64_t comps = n;
+  for (int32_t v = 0; v < n; v++)
+    for (int32_t j = g->nindex[v]; j < g->nindex[v + 1]; j++) {
+      const int32_t u = g->nlist[j];
+      if (u > v) {
+        const int32_t a = uf_find(v, p), b = uf_find(u, p);
+        if (a != b) { uf_union_min(a, b, p); comps--; }
+      }
+    }
+  free(p);
+  *marked = mk;
+  *components = comps;
+  *is_forest = ok;
+  return 0;
+}

@@ -57,6 +57,9 @@ int orc_ecl_schedule(const orc_graph* g, int use_filter, int32_t threshold,
 int orc_verify(const orc_graph* g, const uint8_t* cpu, const uint8_t* gpu,
                uint64_t* cpu_weight, uint64_t* gpu_weight, int64_t* only_cpu, int64_t* only_gpu);
 
+int orc_forest_check(const orc_graph* g, const uint8_t* inMST, int64_t* marked, int64_t* components,
+                     int* is_forest);
+
 /* Parallel CPU baselines (cpu_baselines.c); threads <= 0 keeps the OpenMP default. */
 int orc_boruvka_omp(const orc_graph* g, uint8_t* inMST, int threads);
 int orc_filter_kruskal_omp(const orc_graph* g, uint8_t* inMST, int threads);

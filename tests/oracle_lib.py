@@ -139,6 +139,20 @@ def filter_kruskal_omp(g, threads=0):
     return out[:len(nl)]
 
 
+lib.orc_forest_check.argtypes = [C.POINTER(_G), _pu8, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                 C.POINTER(C.c_int)]
+
+
+def forest_check(g, inmst):
+    """(marked edges, graph components, is_forest) of a candidate forest."""
+    ni, nl, ew = _arrs(g)
+    x = np.ascontiguousarray(inmst, np.uint8)
+    mk, cc, ok = C.c_int64(), C.c_int64(), C.c_int()
+    assert lib.orc_forest_check(C.byref(_view(ni, nl, ew)), x.ctypes.data_as(_pu8), C.byref(mk), C.byref(cc),
+                                C.byref(ok)) == 0
+    return mk.value, cc.value, bool(ok.value)
+
+
 def verify(g, cpu, gpu):
     ni, nl, ew = _arrs(g)
     cpu = np.ascontiguousarray(cpu, np.uint8)

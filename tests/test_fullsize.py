@@ -1,0 +1,82 @@
+"""Full-size configurations on the GPU, through the C-ABI.
+
+C2 RMAT-24 and C3 RGG-2^24: every filter mode against the oracle's serial
+Kruskal, bit for bit.  C5 RMAT-26 with weights in [1, 64] (heavy ties, a
+forest, ~2e9 directed edges): the forest is unique under the (w, j) order,
+so the unfiltered, plain and learned runs must give identical bytes; the
+oracle's union-find then checks that it is a spanning forest, and the
+weights agree with the OpenMP Boruvka restatement.  Set EMST_SKIP_RMAT26=1
+to leave C5 out."""
+import os
+import time
+
+import numpy as np
+import pytest
+
+import emst
+import oracle_lib as O
+
+pytestmark = [pytest.mark.gpu, pytest.mark.slow]
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MODEL = os.path.join(ROOT, "mst-artifact-sc23_amd", "filter_mlp.f32")
+
+
+def _solver():
+    s = emst.Solver(0)
+    s.set_filter_model(np.fromfile(MODEL, np.float32))
+    return s
+
+
+def _modes(s, g):
+    uf, T = emst.plain_threshold(g)
+    s.load(g)
+    out = {}
+    for name, (mode, thr) in (("plain", (emst.FILTER_PLAIN if uf else emst.FILTER_OFF, T)),
+                              ("learned", (emst.FILTER_LEARNED, 0)), ("off", (emst.FILTER_OFF, emst.INT_MAX))):
+        t0 = time.time()
+        st = s.solve(mode, thr)
+        out[name] = (s.fetch(), st)
+        print(f"  {name}: {st.ms:.2f} ms, rounds {st.light_rounds}+{st.heavy_rounds}, light {st.light}, "
+              f"heavy {st.heavy} ({time.time()-t0:.1f}s)", flush=True)
+    return out
+
+
+@pytest.mark.parametrize("cfg", ["rmat24", "rgg24"])
+def test_c2_c3_against_kruskal(cfg):
+    t0 = time.time()
+    g = (emst.gen_rmat(24, 16, 0.57, 0.19, 0.19, 2, 1, 1 << 24) if cfg == "rmat24"
+         else emst.gen_rgg(1 << 24, 6.0, 3, 1 << 24))
+    print(f"{cfg}: n={g.nodes} m={g.edges} generated in {time.time()-t0:.1f}s", flush=True)
+    s = _solver()
+    res = _modes(s, g)
+    s.close()
+    ref = O.kruskal(g)
+    for name, (got, st) in res.items():
+        assert np.array_equal(got, ref), name
+    rc, cw, gw, oc, og = O.verify(g, ref, res["learned"][0])
+    assert rc == 0 and cw == gw and oc == og == 0
+
+
+@pytest.mark.skipif(os.environ.get("EMST_SKIP_RMAT26") == "1", reason="EMST_SKIP_RMAT26=1")
+def test_c5_rmat26_ties_properties():
+    t0 = time.time()
+    g = emst.gen_rmat(26, 16, 0.57, 0.19, 0.19, 5, 1, 65)
+    assert g.edges < 2**31
+    print(f"rmat26: n={g.nodes} m={g.edges} generated in {time.time()-t0:.1f}s", flush=True)
+    s = _solver()
+    res = _modes(s, g)
+    s.close()
+    base = res["off"][0]
+    for name, (got, st) in res.items():
+        assert np.array_equal(got, base), name
+    t1 = time.time()
+    mk, cc, ok = O.forest_check(g, base)
+    print(f"  forest check {time.time()-t1:.1f}s: marked {mk}, components {cc}", flush=True)
+    assert ok and mk == g.nodes - cc
+    assert cc > 1   # a forest: isolated vertices
+    w = emst.mst_weight(g, base)
+    t1 = time.time()
+    b = O.boruvka_omp(g, 0)
+    print(f"  OpenMP Boruvka {time.time()-t1:.1f}s", flush=True)
+    assert emst.mst_weight(g, b) == w
+    assert np.array_equal(b, base)

@@ -264,3 +264,25 @@ def test_parallel_cpu_baselines_match_kruskal():
         k = O.kruskal(g)
         assert np.array_equal(O.boruvka_omp(g, 4), k)
         assert np.array_equal(O.filter_kruskal_omp(g, 4), k)
+
+
+def test_forest_check_pins():
+    import emst
+    rng = np.random.default_rng(31)
+    for _ in range(30):
+        n = int(rng.integers(1, 120))
+        g = G(*random_graph(rng, n, int(rng.integers(0, 300)), 0, 9))
+        k = O.kruskal(g)
+        mk, cc, ok = O.forest_check(g, k)
+        assert ok and mk == n - cc
+        _, st = O.schedule(g, False, 2**31 - 1)
+        assert cc == st.components
+        if k.sum() > 0:                      # drop an edge: no longer spanning
+            bad = k.copy(); bad[np.nonzero(k)[0][0]] = 0
+            mk2, cc2, ok2 = O.forest_check(g, bad)
+            assert ok2 and mk2 == n - cc2 - 1
+        src = np.repeat(np.arange(n), np.diff(g.nindex))
+        extra = np.nonzero((k == 0) & (g.nlist > src))[0]
+        if len(extra):                       # add a non-tree edge: a cycle
+            bad = k.copy(); bad[extra[0]] = 1
+            assert not O.forest_check(g, bad)[2]
