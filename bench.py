@@ -16,7 +16,7 @@ n log-uniform in [1e3, 1e6], generator uniform over grid / geometric deg 6 /
 RMAT ef 8 / Erdos-Renyi deg 10, weights in [1, 2^20), seed 4) and reports
 graphs/s with p50/p99 per-graph latency against the per-graph loop.
 
-Prints one JSON line.  --gpus above 1 is refused: this build measures one
+Prints one JSON line.  --gpus above 1 is refused: the solver runs on one
 GPU.
 """
 import argparse
@@ -325,8 +325,8 @@ def main():
     ap.add_argument("--verify-graphs", type=int, default=32)
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported by this build "
-              "(the path is single-GPU; DESIGN.md describes how it would shard)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "
+              "(the solver runs on one GPU)", file=sys.stderr)
         sys.exit(2)
     if a.workload == "copy":
         print(json.dumps(run_copy(a)))

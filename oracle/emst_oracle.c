@@ -16,8 +16,10 @@
  *                        arithmetic (file size = 8 + 4(n+1) + 8m).
  *   orc_synth_weights  : hand-computed golden vectors incl. int32 wraparound
  *                        and negative-remainder cases (tests/golden).
- *   orc_hash / orc_plain_threshold : hand-computed golden vectors (the hash
- *                        of 0..3 worked by hand, threshold index closed form).
+ *   orc_hash           : known answers computed independently with big
+ *                        integers reduced mod 2^32 (tests/golden/golden.json).
+ *   orc_plain_threshold: closed-form sample indices worked by hand for
+ *                        (n, m) = (10, 40), (100, 6000), (1, 4) and m/n < 4.
  *   orc_kruskal        : brute-force MSF enumeration for n <= 7, closed forms
  *                        (trees, all-equal weights, isolated vertices), the
  *                        independent Boruvka/filter-Kruskal in cpu_baselines.c.
@@ -26,6 +28,8 @@
  *                        unique); per-round sizes checked against a hand
  *                        trace of a path graph and against invariants.
  *   orc_verify         : direct cases (equal, one flipped bit, weight sum).
+ *   orc_forest_check   : Kruskal's forest passes; dropping an edge breaks
+ *                        spanning, adding a non-tree edge makes a cycle.
  */
 #include <limits.h>
 #include <stdint.h>
