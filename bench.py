@@ -226,6 +226,10 @@ def run_batch(a, emst):
     s = emst.Solver(0)
     s.set_filter_model(np.fromfile(os.path.join(PKG, "filter_mlp.f32"), np.float32))
     b = emst.Batch(s, graphs)
+    union_model = os.path.join(PKG, "filter_mlp_union.f32")
+    if os.path.exists(union_model):
+        # the network trained for graphs that share an arena's rounds
+        b.set_filter_model(np.fromfile(union_model, np.float32))
     for _ in range(a.warmup):
         b.solve(emst.FILTER_LEARNED, use_graph=True)
     bt = []
@@ -271,6 +275,8 @@ def run_batch(a, emst):
                              if len(bt) >= 2 else batch_s * 1e3,
                              "note": "every graph of a batch completes with the batch"},
         "batch_plain_ms": statistics.median(plain_b) * 1e3,
+        "learned_vs_plain": statistics.median(plain_b) / batch_s,
+        "network": os.path.basename(union_model) if os.path.exists(union_model) else "filter_mlp.f32",
         "per_graph_loop": {"graphs_per_s": len(graphs) / loop_total_s, "p50_ms": float(np.percentile(lat, 50)) * 1e3,
                            "p99_ms": float(np.percentile(lat, 99)) * 1e3, "sampled_graphs": len(sample),
                            "total_s_estimated": loop_total_s},

@@ -207,6 +207,12 @@ int emst_batch_solve(emst_batch* b, int use_filter, const int32_t* thresholds, i
 int emst_batch_fetch(emst_batch* b, int32_t g, uint8_t* inMST);
 /* Per-graph thresholds and filter flags of the last solve. */
 int emst_batch_thresholds(const emst_batch* b, int32_t* thresholds, int32_t* use_filter);
+/* Network used by EMST_FILTER_LEARNED for this batch (EMST_MLP_PARAMS fp32
+ * values, as emst_set_filter_model); NULL falls back to the context's.  The
+ * rounds of an arena are shared by all its graphs, so a graph's cost there
+ * is its bytes, not its round count: a network trained for that cost
+ * (filter_mlp_union.f32) belongs here. */
+int emst_batch_set_filter_model(emst_batch* b, const float* params);
 
 /* ---- learned light-edge threshold (no counterpart in the reference) -----
  * A 32-64-64-1 network predicts log2 of the light-edge quantile from 32

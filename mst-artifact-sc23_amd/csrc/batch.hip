@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -64,6 +65,8 @@ struct emst_batch {
   std::vector<int32_t> last_thr, last_flag;
   long long total_nodes = 0, total_edges = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  unsigned short* d_mlp16 = nullptr;  // the batch's own network (bf16), if set
+  int have_model = 0;
 };
 
 namespace {
@@ -220,7 +223,30 @@ void emst_batch_destroy(emst_batch* b)
   for (Chunk& c : b->chunks) free_chunk(c);
   if (b->e0) (void)hipEventDestroy(b->e0);
   if (b->e1) (void)hipEventDestroy(b->e1);
+  if (b->d_mlp16) (void)hipFree(b->d_mlp16);
   delete b;
+}
+
+int emst_batch_set_filter_model(emst_batch* b, const float* params)
+{
+  if (!b) return EMST_E_ARG;
+  emst_ctx* ctx = b->ctx;
+  if (!params) {
+    b->have_model = 0;
+    return EMST_OK;
+  }
+  std::vector<uint16_t> h(kParams);
+  for (int i = 0; i < kParams; i++) {
+    if (!std::isfinite(params[i])) return ctx_fail(ctx, EMST_E_ARG, "non-finite model parameter");
+    h[i] = emst_bf16_bits(params[i]);
+  }
+  if (hipSetDevice(ctx->device) != hipSuccess) return ctx_fail(ctx, EMST_E_HIP, "hipSetDevice failed");
+  if (!b->d_mlp16 && hipMalloc((void**)&b->d_mlp16, kParams * 2) != hipSuccess)
+    return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of the batch network failed");
+  if (hipMemcpy(b->d_mlp16, h.data(), kParams * 2, hipMemcpyHostToDevice) != hipSuccess)
+    return ctx_fail(ctx, EMST_E_HIP, "upload of the batch network failed");
+  b->have_model = 1;
+  return EMST_OK;
 }
 
 int emst_batch_info(const emst_batch* b, int32_t* chunks, int64_t* total_nodes, int64_t* total_edges)
@@ -247,8 +273,11 @@ int emst_batch_solve(emst_batch* b, int use_filter, const int32_t* thresholds, i
     // per-graph thresholds
     std::vector<int32_t> thr(G), flag(G);
     if (use_filter == EMST_FILTER_LEARNED) {
-      launch_learned_batch(s, G, c.d_nindex, c.d_eweight, c.d_voff, c.d_eoff, ctx->d_mlp16, ctx->have_model, c.d_gthr,
-                           c.d_gflag);
+      if (b->have_model)
+        launch_learned_batch(s, G, c.d_nindex, c.d_eweight, c.d_voff, c.d_eoff, b->d_mlp16, 1, c.d_gthr, c.d_gflag);
+      else
+        launch_learned_batch(s, G, c.d_nindex, c.d_eweight, c.d_voff, c.d_eoff, ctx->d_mlp16, ctx->have_model,
+                             c.d_gthr, c.d_gflag);
       if (hipMemcpyAsync(thr.data(), c.d_gthr, (size_t)G * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
           hipMemcpyAsync(flag.data(), c.d_gflag, (size_t)G * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
           hipStreamSynchronize(s) != hipSuccess)

@@ -28,7 +28,7 @@ struct emst_ctx {
   // work buffers, grown on demand and reused
   int* d_parent = nullptr;
   emst::u64* d_minv = nullptr;
-  unsigned* d_bigmap = nullptr;
+  unsigned* d_vclass = nullptr;
   size_t cap_nodes = 0;
   int4* d_wl[2] = {nullptr, nullptr};
   size_t cap_wl = 0;

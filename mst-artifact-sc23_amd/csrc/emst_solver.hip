@@ -13,7 +13,7 @@
 //                      (kernel2 + kernel3, ECL_MST.cu:229-251, fused)
 //                      k_select: drop records inside one component, write
 //                      back roots, offer keys (kernel1, ECL_MST.cu:210-226)
-//   k_pick_big, k_flatten  parent[v] = root(v); bitmap of one big component
+//   k_pick_big, k_flatten  parent[v] = root(v); class map (big component, roots)
 //   k_init_wl<heavy>   heavy upper-direction edges whose roots differ
 //                      (initWL<false>, ECL_MST.cu:157-207, launch :326)
 //   rounds             as above
@@ -104,10 +104,10 @@ __global__ __launch_bounds__(256) void k_zero(uint8_t* __restrict__ p, long long
 //                reference also tests w > T (ECL_MST.cu:197); after the
 //                light phase every light edge lies inside one component, so
 //                "roots differ" already implies w > T and the weight is
-//                read only for crossing edges.  bigmap marks the members of
-//                one large component (2 MB for 2^24 vertices, resident in
-//                L2); an edge with both ends marked is skipped without
-//                touching the root array.
+//                read only for crossing edges.  The vertex class map (two
+//                bits per vertex, 4 MB for 2^24 vertices, mostly L2
+//                resident) marks the members of one large component and the
+//                roots, whose root is known without touching parent.
 // One wave task covers SUB*64 consecutive CSR positions.  Lane i takes
 // positions base+i, base+64+i, ... so every load is one coalesced 256-byte
 // request, and all SUB loads of a task are issued before the first use.
@@ -123,7 +123,7 @@ template <bool HEAVY, int SUB>
 __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
     const int* __restrict__ nindex, const int* __restrict__ nlist, const int* __restrict__ eweight,
     const int* __restrict__ gsrc, int nodes, int edges, int T, const int* __restrict__ label,
-    const unsigned* __restrict__ bigmap, int4* __restrict__ wl, int* __restrict__ cnt, int stage_cap,
+    const unsigned* __restrict__ vclass, int4* __restrict__ wl, int* __restrict__ cnt, int stage_cap,
     long long* __restrict__ stats, const int* __restrict__ dthr, const int* __restrict__ vgid,
     const int* __restrict__ gthr, const int* __restrict__ bigroot_p)
 {
@@ -184,33 +184,33 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
         stage_push(st, keep, make_int4(src[u], nl[u], w[u], j), wl, cnt);
       }
     } else {
-      // bit 0: upper direction, bit 1: source marked, bit 2: target marked
-      int flag[SUB];
-#pragma unroll
-      for (int u = 0; u < SUB; u++) {
-        flag[u] = 0;
-        if (nl[u] > src[u]) {
-          const unsigned bu = (bigmap[(unsigned)nl[u] >> 5] >> (nl[u] & 31)) & 1u;
-          const unsigned bv = (bigmap[(unsigned)src[u] >> 5] >> (src[u] & 31)) & 1u;
-          flag[u] = 1 | (int)(bv << 1) | (int)(bu << 2);
-        }
-      }
-      // a marked endpoint's root is the marked component's root, so only
-      // unmarked endpoints need a root lookup
+      // The class map resolves most roots without touching parent: a
+      // member of the marked component has root bigroot, a root is its own
+      // root; only the remaining endpoints (inside other non-trivial
+      // components) are looked up.
+      bool up[SUB];
       int a[SUB], b[SUB];
 #pragma unroll
       for (int u = 0; u < SUB; u++) {
+        up[u] = nl[u] > src[u];
         a[u] = bigroot;
         b[u] = bigroot;
-        if ((flag[u] & 1) && (flag[u] & 6) != 6) {
-          if (!(flag[u] & 2)) a[u] = label[src[u]];
-          if (!(flag[u] & 4)) b[u] = label[nl[u]];
+        if (up[u]) {
+          const unsigned cv = (vclass[(unsigned)src[u] >> 4] >> ((src[u] & 15) * 2)) & 3u;
+          const unsigned cu = (vclass[(unsigned)nl[u] >> 4] >> ((nl[u] & 15) * 2)) & 3u;
+          a[u] = (cv & 1u) ? bigroot : (cv & 2u) ? src[u] : -1;
+          b[u] = (cu & 1u) ? bigroot : (cu & 2u) ? nl[u] : -1;
         }
+      }
+#pragma unroll
+      for (int u = 0; u < SUB; u++) {
+        if (a[u] < 0) a[u] = label[src[u]];
+        if (b[u] < 0) b[u] = label[nl[u]];
       }
 #pragma unroll
       for (int u = 0; u < SUB; u++) {
         const long long p = gbeg + u * kWave + lane;
-        const bool keep = (flag[u] & 1) && a[u] != b[u];
+        const bool keep = up[u] && a[u] != b[u];
         const int wt = keep ? eweight[p] : 0;
         emitted += keep;
         stage_push(st, keep, make_int4(a[u], b[u], wt, (int)p), wl, cnt);
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
   if (emitted) atomicAdd((unsigned long long*)&stats[HEAVY ? ST_HEAVY : ST_LIGHT], (unsigned long long)emitted);
 }
 
-// Picks the component that the heavy pass' bitmap marks: the most frequent
+// Picks the component that the heavy pass' class map marks: the most frequent
 // root among 64 evenly spaced vertices (one wave).  Any choice is correct;
 // a frequent one saves the most root lookups.
 __global__ __launch_bounds__(64) void k_pick_big(int nodes, const int* __restrict__ parent, int* __restrict__ out,
@@ -241,10 +241,22 @@ __global__ __launch_bounds__(64) void k_pick_big(int nodes, const int* __restric
   if (lane == 0) out[0] = big;
 }
 
-// parent[v] = root(v), and bit v of bigmap = (root(v) == *big).  Concurrent
-// writers only shorten chains toward the same root, so readers stay correct.
+// Spreads the 16 low bits of x to the even bit positions of a word.
+__device__ __forceinline__ unsigned spread16(unsigned x)
+{
+  x &= 0xffffu;
+  x = (x | (x << 8)) & 0x00ff00ffu;
+  x = (x | (x << 4)) & 0x0f0f0f0fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  return (x | (x << 1)) & 0x55555555u;
+}
+
+// parent[v] = root(v), and the vertex class map: two bits per vertex,
+// sixteen vertices per word, bit 0 = (root(v) == *big), bit 1 = (v is a
+// root).  Concurrent writers only shorten chains toward the same root, so
+// readers stay correct.
 __global__ __launch_bounds__(256) void k_flatten(int nodes, int* __restrict__ parent, int* __restrict__ cnt,
-                                                 const int* __restrict__ big, unsigned* __restrict__ bigmap,
+                                                 const int* __restrict__ big, unsigned* __restrict__ vclass,
                                                  const int* __restrict__ dthr)
 {
   const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -259,13 +271,14 @@ __global__ __launch_bounds__(256) void k_flatten(int nodes, int* __restrict__ pa
     r = find_root((int)v, parent);
     parent[v] = r;
   }
-  const u64 m = __ballot(r == bigroot && r >= 0);
-  const long long v0 = v - lane_id();
-  const long long words = ((long long)nodes + 31) / 32;
-  if (lane_id() == 0 && v0 < nodes) {
-    bigmap[v0 >> 5] = (unsigned)m;
-    if ((v0 >> 5) + 1 < words) bigmap[(v0 >> 5) + 1] = (unsigned)(m >> 32);
-  }
+  const u64 mb = __ballot(r == bigroot && r >= 0);
+  const u64 mr = __ballot(r >= 0 && r == (int)v);
+  const int lane = lane_id();
+  const long long v0 = v - lane;
+  const long long words = ((long long)nodes + 15) / 16;
+  const long long wi = (v0 >> 4) + lane;
+  if (lane < 4 && v0 < nodes && wi < words)
+    vclass[wi] = spread16((unsigned)(mb >> (16 * lane))) | (spread16((unsigned)(mr >> (16 * lane))) << 1);
 }
 
 // Offers key to rep: min-combined in the workgroup's LDS table when the rep
@@ -616,14 +629,14 @@ namespace emst {
 int ensure_buffers(emst_ctx* ctx, int32_t nodes, int32_t edges, long long wl_cap, int task_edges)
 {
   if ((size_t)nodes > ctx->cap_nodes || !ctx->d_parent) {
-    for (void** p : {(void**)&ctx->d_parent, (void**)&ctx->d_minv, (void**)&ctx->d_bigmap}) {
+    for (void** p : {(void**)&ctx->d_parent, (void**)&ctx->d_minv, (void**)&ctx->d_vclass}) {
       if (*p) (void)hipFree(*p);
       *p = nullptr;
     }
     ctx->cap_nodes = 0;
     const size_t n = (size_t)std::max(nodes, 1);
     if (hipMalloc((void**)&ctx->d_parent, n * 4) != hipSuccess || hipMalloc((void**)&ctx->d_minv, n * 8) != hipSuccess ||
-        hipMalloc((void**)&ctx->d_bigmap, (n + 63) / 32 * 4) != hipSuccess)
+        hipMalloc((void**)&ctx->d_vclass, (n + 63) / 16 * 4) != hipSuccess)
       return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of parent/minv failed");
     ctx->cap_nodes = n;
     ctx->buf_gen++;
@@ -720,13 +733,13 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       const int blocks = (int)(((long long)nodes + 255) / 256);
       hipLaunchKernelGGL(k_pick_big, dim3(1), dim3(kWave), 0, s, nodes, ctx->d_parent, &d_cnt[2], dthr);
       hipLaunchKernelGGL(k_flatten, dim3(blocks), dim3(256), 0, s, nodes, ctx->d_parent, d_cnt, &d_cnt[2],
-                         ctx->d_bigmap, dthr);
+                         ctx->d_vclass, dthr);
     }
     if (edges > 0) {
       if (!fixed) HIPCHK(hipEventRecord(ctx->ev[2 + 2 * phase], s));
       const InitFn f = init_fn(phase == 1, sub);
       hipLaunchKernelGGL(f, dim3(init_grid), dim3(kInitWaves * kWave), init_lds, s, d_nindex, d_nlist, d_eweight,
-                         ctx->d_gsrc, nodes, edges, T, ctx->d_parent, ctx->d_bigmap, wl[0], &d_cnt[0], stage_cap,
+                         ctx->d_gsrc, nodes, edges, T, ctx->d_parent, ctx->d_vclass, wl[0], &d_cnt[0], stage_cap,
                          d_stats, dthr, phase == 0 ? vgid : nullptr, phase == 0 ? gthr : nullptr, &d_cnt[2]);
       if (!fixed) HIPCHK(hipEventRecord(ctx->ev[3 + 2 * phase], s));
       HIPCHK(hipGetLastError());
@@ -897,7 +910,7 @@ void emst_destroy(emst_ctx* ctx)
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   void* ptrs[] = {ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, ctx->d_inMST, ctx->d_parent,
                   ctx->d_minv,   ctx->d_wl[0], ctx->d_wl[1],   ctx->d_gsrc,  ctx->d_cnt,
-                  ctx->d_stats,  ctx->d_bigmap, ctx->d_mlp16,  ctx->d_thr,   ctx->d_q,
+                  ctx->d_stats,  ctx->d_vclass, ctx->d_mlp16,  ctx->d_thr,   ctx->d_q,
                   ctx->d_feat,   ctx->d_samples};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

@@ -144,6 +144,7 @@ _batch_destroy = _sig("emst_batch_destroy", None, C.c_void_p)
 _batch_solve = _sig("emst_batch_solve", C.c_int, C.c_void_p, C.c_int, _p32, C.c_int, C.POINTER(BatchStats))
 _batch_fetch = _sig("emst_batch_fetch", C.c_int, C.c_void_p, C.c_int32, _pu8)
 _batch_thresholds = _sig("emst_batch_thresholds", C.c_int, C.c_void_p, _p32, _p32)
+_batch_set_model = _sig("emst_batch_set_filter_model", C.c_int, C.c_void_p, _pf32)
 _batch_info = _sig("emst_batch_info", C.c_int, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                    C.POINTER(C.c_int64))
 _autotune = _sig("emst_autotune", C.c_int, C.c_void_p, C.c_int, C.c_int32, C.c_int32, C.POINTER(Config),
@@ -486,6 +487,16 @@ class Batch:
         out = np.zeros(max(m, 1), np.uint8)
         self.solver._err(_batch_fetch(self._h, int(g), out.ctypes.data_as(_pu8)))
         return out[:m]
+
+    def set_filter_model(self, params):
+        """Network for FILTER_LEARNED solves of this batch; None uses the solver's."""
+        if params is None:
+            self.solver._err(_batch_set_model(self._h, None))
+            return
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.shape == (MLP_PARAMS,)
+        self._model = p
+        self.solver._err(_batch_set_model(self._h, p.ctypes.data_as(_pf32)))
 
     def thresholds(self):
         t = np.zeros(len(self.graphs), np.int32)

@@ -10,10 +10,18 @@ solves (needs a HIP device).
        cost = LAUNCH_US * rounds + bytes / BW
    where bytes is the schedule's byte count (initialisation passes,
    records, 24 B per record entering a round, 96 B per survivor).  The
-   label is log2 of the cheapest q.
-3. Full-batch Adam on the device (emst_mlp_train), then an evaluation on
-   held-out graphs against the plain rule (ECL_MST.cu:432-443).
-Writes <out>/filter_mlp.f32 (fp32 parameters) and <out>/filter_mlp.json.
+   label is log2 of the cheapest q.  Two cost models give two label sets:
+     single  LAUNCH_US per round: one graph per solve, so every round's
+             fixed cost is the graph's own;
+     union   LAUNCH_US / UNION_SHARE per round: in a batched arena about
+             UNION_SHARE graphs share every round, so a graph's marginal
+             cost is its bytes.
+3. Full-batch Adam on the device (emst_mlp_train), one network per cost
+   model, then an evaluation on held-out graphs against the plain rule
+   (ECL_MST.cu:432-443) under the same cost model.
+Writes <out>/filter_mlp.f32, <out>/filter_mlp_union.f32 (fp32 parameters),
+<out>/filter_costs.npz (features and the per-candidate rounds and bytes of
+every training graph) and <out>/filter_mlp.json.
 """
 import argparse
 import json
@@ -29,6 +37,7 @@ import emst  # noqa: E402
 
 QS = [2.0 ** (-k / 2.0) for k in range(14, 0, -1)] + [1.0]   # 2^-7 .. 2^-0.5, then unfiltered
 LAUNCH_US = 8.0
+UNION_SHARE = 1000.0
 BW = 3.0e12
 
 
@@ -64,9 +73,12 @@ def schedule_bytes(g, st, m_up_heavy):
     return b
 
 
-def cost_of(g, st, m_up_heavy):
-    rounds = st.light_rounds + st.heavy_rounds
-    return LAUNCH_US * rounds + schedule_bytes(g, st, m_up_heavy) / BW * 1e6
+def cost_from(rounds, nbytes, launch_us):
+    return launch_us * rounds + nbytes / BW * 1e6
+
+
+def cost_of(g, st, m_up_heavy, launch_us=LAUNCH_US):
+    return cost_from(st.light_rounds + st.heavy_rounds, schedule_bytes(g, st, m_up_heavy), launch_us)
 
 
 def label_graph(solver, g):
@@ -82,17 +94,41 @@ def label_graph(solver, g):
             T = int(samples[int(math.floor(q * 1023))])
             st = solver.solve(emst.FILTER_PLAIN, T)
             heavy_up = int(len(upw) - np.searchsorted(upw, T, side="right"))
-        rows.append((q, cost_of(g, st, heavy_up), st.ms, st.light_rounds + st.heavy_rounds))
-    best = min(rows, key=lambda r: r[1])
-    return feat, best[0], rows
+        rows.append((st.light_rounds + st.heavy_rounds, schedule_bytes(g, st, heavy_up)))
+    return feat, rows
 
 
-def plain_cost(solver, g):
+def best_q(rows, launch_us):
+    costs = [cost_from(r, b, launch_us) for r, b in rows]
+    return QS[int(np.argmin(costs))]
+
+
+def plain_cost(solver, g, launch_us):
     uf, T = emst.plain_threshold(g)
     st = solver.solve(emst.FILTER_PLAIN if uf else emst.FILTER_OFF, T)
     upw = np.sort(g.eweight[g.upper_mask()])
     hu = int(len(upw) - np.searchsorted(upw, T, side="right")) if uf else 0
-    return cost_of(g, st, hu), st
+    return cost_of(g, st, hu, launch_us), st
+
+
+def evaluate(solver, params, launch_us, holdout, seed):
+    """Modelled cost of the learned and the plain thresholds on held-out graphs."""
+    solver.set_filter_model(params)
+    ev = []
+    for i in range(holdout):
+        kind, g = make_graph(10_000_000 + i, seed + 1)
+        solver.load(g)
+        pc, pst = plain_cost(solver, g, launch_us)
+        lst = solver.solve(emst.FILTER_LEARNED, 0)
+        upw = np.sort(g.eweight[g.upper_mask()])
+        hu = int(len(upw) - np.searchsorted(upw, lst.threshold_used, side="right")) if lst.filter_used else 0
+        lc = cost_of(g, lst, hu, launch_us)
+        ev.append({"kind": kind, "n": g.nodes, "m": g.edges, "plain_cost": pc, "learned_cost": lc,
+                   "plain_rounds": pst.light_rounds + pst.heavy_rounds,
+                   "learned_rounds": lst.light_rounds + lst.heavy_rounds, "plain_light": pst.light,
+                   "learned_light": lst.light, "q": lst.learned_q})
+    ratio = float(np.exp(np.mean([math.log(e["learned_cost"] / e["plain_cost"]) for e in ev])))
+    return ratio, ev
 
 
 def main():
@@ -103,53 +139,53 @@ def main():
     ap.add_argument("--epochs", type=int, default=4000)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", default="gpurun_out/filter")
+    ap.add_argument("--costs", default=None, help="reuse a filter_costs.npz instead of solving the graphs again")
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
     solver = emst.Solver(0)
-    X, Y, meta = [], [], []
     t0 = time.time()
-    for i in range(a.graphs):
-        kind, g = make_graph(i, a.seed)
-        feat, qbest, rows = label_graph(solver, g)
-        X.append(feat)
-        Y.append(math.log2(qbest))
-        meta.append({"kind": kind, "n": g.nodes, "m": g.edges, "q": qbest})
-        if i % 50 == 0:
-            print(f"[train_filter] labelled {i}/{a.graphs} ({time.time()-t0:.0f}s) last {kind} n={g.nodes} q*={qbest:.4f}",
-                  flush=True)
-    X = np.array(X, np.float32)
-    Y = np.array(Y, np.float32)
-    np.save(os.path.join(a.out, "train_X.npy"), X)
-    np.save(os.path.join(a.out, "train_Y.npy"), Y)
-    p0 = emst.init_params(a.seed)
-    t1 = time.time()
-    params, loss = emst.mlp_train(p0, X, Y, a.epochs, a.lr, a.seed)
-    print(f"[train_filter] trained {a.epochs} epochs in {time.time()-t1:.1f}s, final mse {loss:.4f} "
-          f"(label variance {float(np.var(Y)):.4f})", flush=True)
-    params.astype(np.float32).tofile(os.path.join(a.out, "filter_mlp.f32"))
-    # held-out evaluation against the plain rule
-    solver.set_filter_model(params)
-    ev = []
-    for i in range(a.holdout):
-        kind, g = make_graph(10_000_000 + i, a.seed + 1)
-        solver.load(g)
-        pc, pst = plain_cost(solver, g)
-        lst = solver.solve(emst.FILTER_LEARNED, 0)
-        upw = np.sort(g.eweight[g.upper_mask()])
-        hu = int(len(upw) - np.searchsorted(upw, lst.threshold_used, side="right")) if lst.filter_used else 0
-        lc = cost_of(g, lst, hu)
-        ev.append({"kind": kind, "n": g.nodes, "m": g.edges, "plain_cost": pc, "learned_cost": lc,
-                   "plain_rounds": pst.light_rounds + pst.heavy_rounds,
-                   "learned_rounds": lst.light_rounds + lst.heavy_rounds, "plain_light": pst.light,
-                   "learned_light": lst.light, "q": lst.learned_q})
-    ratio = float(np.exp(np.mean([math.log(e["learned_cost"] / e["plain_cost"]) for e in ev])))
-    info = {"params": emst.MLP_PARAMS, "graphs": a.graphs, "seed": a.seed, "epochs": a.epochs, "lr": a.lr,
-            "final_mse": loss, "label_variance": float(np.var(Y)), "candidates": QS, "launch_us": LAUNCH_US,
-            "bw": BW, "holdout_geomean_cost_ratio": ratio, "holdout": ev, "train_meta": meta,
-            "seconds": time.time() - t0}
+    if a.costs:
+        z = np.load(a.costs)
+        X, R, B = z["X"], z["rounds"], z["bytes"]
+        kinds = [str(k) for k in z["kinds"]]
+    else:
+        X, R, B, kinds = [], [], [], []
+        for i in range(a.graphs):
+            kind, g = make_graph(i, a.seed)
+            feat, rows = label_graph(solver, g)
+            X.append(feat)
+            R.append([r for r, _ in rows])
+            B.append([b for _, b in rows])
+            kinds.append(kind)
+            if i % 50 == 0:
+                print(f"[train_filter] labelled {i}/{a.graphs} ({time.time()-t0:.0f}s) last {kind} n={g.nodes}",
+                      flush=True)
+        X = np.array(X, np.float32)
+        R = np.array(R, np.int64)
+        B = np.array(B, np.float64)
+        np.savez(os.path.join(a.out, "filter_costs.npz"), X=X, rounds=R, bytes=B, kinds=np.array(kinds),
+                 qs=np.array(QS))
+    info = {"params": emst.MLP_PARAMS, "graphs": int(len(X)), "seed": a.seed, "epochs": a.epochs, "lr": a.lr,
+            "candidates": QS, "bw": BW, "models": {}}
+    for name, launch_us, fname in (("single", LAUNCH_US, "filter_mlp.f32"),
+                                   ("union", LAUNCH_US / UNION_SHARE, "filter_mlp_union.f32")):
+        qs = [best_q(list(zip(R[i], B[i])), launch_us) for i in range(len(X))]
+        Y = np.log2(np.array(qs, np.float32))
+        p0 = emst.init_params(a.seed)
+        t1 = time.time()
+        params, loss = emst.mlp_train(p0, X, Y, a.epochs, a.lr, a.seed)
+        print(f"[train_filter] {name}: trained {a.epochs} epochs in {time.time()-t1:.1f}s, final mse {loss:.4f} "
+              f"(label variance {float(np.var(Y)):.4f})", flush=True)
+        params.astype(np.float32).tofile(os.path.join(a.out, fname))
+        ratio, ev = evaluate(solver, params, launch_us, a.holdout, a.seed)
+        print(f"[train_filter] {name}: held-out geomean learned/plain modelled cost = {ratio:.3f}", flush=True)
+        info["models"][name] = {"file": fname, "launch_us": launch_us, "final_mse": loss,
+                                "label_variance": float(np.var(Y)), "holdout_geomean_cost_ratio": ratio,
+                                "holdout": ev, "label_q_histogram": {str(q): int(sum(1 for x in qs if x == q))
+                                                                     for q in QS}}
+    info["seconds"] = time.time() - t0
     with open(os.path.join(a.out, "filter_mlp.json"), "w") as f:
         json.dump(info, f)
-    print(f"[train_filter] held-out geomean learned/plain modelled cost = {ratio:.3f}", flush=True)
 
 
 if __name__ == "__main__":

@@ -18,6 +18,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import mlp_oracle as M  # noqa: E402
 
 MODEL = os.path.join(ROOT, "mst-artifact-sc23_amd", "filter_mlp.f32")
+MODEL_UNION = os.path.join(ROOT, "mst-artifact-sc23_amd", "filter_mlp_union.f32")
 
 
 def _params(seed=0):
@@ -45,10 +46,11 @@ def test_bf16_rounding_matches_library():
     assert np.array_equal(emst.bf16_round(vals), M.bf16(vals))
 
 
-def test_committed_model_file_shape():
-    if not os.path.exists(MODEL):
+@pytest.mark.parametrize("path", [MODEL, MODEL_UNION])
+def test_committed_model_file_shape(path):
+    if not os.path.exists(path):
         pytest.skip("no trained model committed")
-    p = np.fromfile(MODEL, np.float32)
+    p = np.fromfile(path, np.float32)
     assert p.shape == (emst.MLP_PARAMS,) and np.all(np.isfinite(p))
 
 
@@ -115,28 +117,34 @@ def test_learned_mode_is_exact_for_any_model():
 
 
 @gpu
-def test_trained_model_beats_plain_rule():
-    if not os.path.exists(MODEL):
+@pytest.mark.parametrize("which", ["single", "union"])
+def test_trained_model_beats_plain_rule(which):
+    """The committed networks against the plain rule, each under the cost
+    model it was trained for: a solve of its own (per-round cost counts) or
+    a graph inside a batched arena (rounds shared, bytes count)."""
+    path = MODEL if which == "single" else MODEL_UNION
+    if not os.path.exists(path):
         pytest.skip("no trained model committed")
     sys.path.insert(0, os.path.join(ROOT, "mst-artifact-sc23_amd"))
     import train_filter as TF
+    launch_us = TF.LAUNCH_US if which == "single" else TF.LAUNCH_US / TF.UNION_SHARE
     s = emst.Solver(0)
-    s.set_filter_model(np.fromfile(MODEL, np.float32))
+    s.set_filter_model(np.fromfile(path, np.float32))
     ratios = []
     for i in range(24):
         kind, g = TF.make_graph(20_000_000 + i, 99)
         s.load(g)
-        pc, pst = TF.plain_cost(s, g)
+        pc, pst = TF.plain_cost(s, g, launch_us)
         got, lst = s.gpu_mst(g, emst.FILTER_LEARNED, 0)
         assert np.array_equal(got, O.kruskal(g))
         upw = np.sort(g.eweight[g.upper_mask()])
         hu = int(len(upw) - np.searchsorted(upw, lst.threshold_used, side="right")) if lst.filter_used else 0
-        lc = TF.cost_of(g, lst, hu)
+        lc = TF.cost_of(g, lst, hu, launch_us)
         print(f"{kind} n={g.nodes} plain {pc:.1f} learned {lc:.1f} q={lst.learned_q:.4f} "
               f"light {pst.light}->{lst.light} rounds {pst.light_rounds + pst.heavy_rounds}->"
               f"{lst.light_rounds + lst.heavy_rounds}", flush=True)
         ratios.append(lc / pc)
     s.close()
     gm = math.exp(np.mean(np.log(ratios)))
-    print("geomean learned/plain modelled cost", gm)
+    print(which, "geomean learned/plain modelled cost", gm)
     assert gm < 0.95
