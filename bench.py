@@ -75,7 +75,8 @@ def cpu_baseline(emst, w, sample_scale, g_full):
     t0 = time.perf_counter()
     ref = oracle_lib.kruskal(g)
     dt = time.perf_counter() - t0
-    cores = os.cpu_count()
+    # the host cores this process may use, at most the 16 a one-GPU share holds
+    cores = min(len(os.sched_getaffinity(0)), 16)
     par = {}
     for name, fn in (("openmp_boruvka", oracle_lib.boruvka_omp), ("openmp_filter_kruskal", oracle_lib.filter_kruskal_omp)):
         t1 = time.perf_counter()
@@ -83,7 +84,13 @@ def cpu_baseline(emst, w, sample_scale, g_full):
         d = time.perf_counter() - t1
         par[name] = {"value": g.edges / d / 1e9, "unit": "Gedges/s", "cores": cores, "seconds": d,
                      "exact": bool(np.array_equal(got, ref))}
-    return {"value": g.edges / dt / 1e9, "unit": "Gedges/s", "cores": 1, "kind": "port",
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+    except OSError:
+        pass
+    return {"value": g.edges / dt / 1e9, "unit": "Gedges/s", "cores": 1, "kind": "port", "cpu_model": model,
             "sample": f"oracle serial Kruskal on {what}: m={g.edges} directed edges in {dt:.2f} s",
             "parallel_restatements": par}
 
@@ -159,6 +166,20 @@ def run_single(a, emst):
     light_roof = {"kernel": "k_init_wl<light>", "achieved": light_bytes / (light_ms * 1e-3) / 1e9,
                   "frac": light_bytes / (light_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, "alg_bytes_per_launch": light_bytes,
                   "avg_launch_ms": light_ms, "traffic": tr.get("k_init_wl<light>") if tr else None}
+    # Whole-solve byte model of the reference schedule (every logical access
+    # once at its natural size) and the compulsory bytes (read the CSR once,
+    # write inMST once), both against the median device time.
+    mst_edges = int(np.count_nonzero(learned_inmst))
+    P = 2 if stl.filter_used else 1
+    model = 12 * n + m + P * (4 * (n + 1) + 6 * m) + 16 * stl.light
+    if stl.filter_used:
+        model += 4 * n + 4 * heavy_up + 16 * stl.heavy
+    model += 24 * stl.round_in_sum + 96 * stl.round_out_sum + 5 * mst_edges
+    compulsory = 9 * m + 4 * (n + 1)
+    t_med = statistics.median(x.ms for x in st_l) * 1e-3
+    solve_roof = {"model_bytes": model, "model_GBps": model / t_med / 1e9, "model_frac": model / t_med / 1e9 / PEAK_HBM_GBS,
+                  "compulsory_bytes": compulsory, "compulsory_GBps": compulsory / t_med / 1e9,
+                  "compulsory_frac": compulsory / t_med / 1e9 / PEAK_HBM_GBS, "median_device_ms": t_med * 1e3}
     out = {
         "metric": "MST Gedges/s at exact weight (directed CSR edges / device-resident solve), learned filter",
         "value": m / (ms_learned * 1e-3) / 1e9,
@@ -186,6 +207,7 @@ def run_single(a, emst):
                    "solver_config": s.config().as_dict()},
         "roofline": dom,
         "roofline_light_init": light_roof,
+        "solve_bytes": solve_roof,
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(emst, w, a.cpu_sample_scale, g)

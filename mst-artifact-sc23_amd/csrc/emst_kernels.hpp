@@ -36,6 +36,17 @@ enum {
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
+// Streaming accesses that are read or written once per pass are issued
+// non-temporal, so they do not evict the randomly accessed per-vertex
+// arrays from L2 and the memory-side cache.
+typedef int v4i __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ int ld_stream(const int* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void st_stream(int4* p, int4 r)
+{
+  v4i v = {r.x, r.y, r.z, r.w};
+  __builtin_nontemporal_store(v, reinterpret_cast<v4i*>(p));
+}
+
 // Number of set bits of m in lanes below this one.
 __device__ __forceinline__ int lanes_below(u64 m)
 {
@@ -123,7 +134,7 @@ __device__ __forceinline__ void stage_flush(Stage& s, int4* __restrict__ out, in
   int base = 0;
   if (lane_id() == 0) base = atomicAdd(cnt, s.n);
   base = __shfl(base, 0);
-  for (int i = lane_id(); i < s.n; i += kWave) out[(long long)base + i] = s.buf[i];
+  for (int i = lane_id(); i < s.n; i += kWave) st_stream(&out[(long long)base + i], s.buf[i]);
   wave_lds_sync();
   s.n = 0;
 }

@@ -110,7 +110,8 @@ __global__ __launch_bounds__(256) void k_zero(uint8_t* __restrict__ p, long long
 //                roots, whose root is known without touching parent.
 // One wave task covers SUB*64 consecutive CSR positions.  Lane i takes
 // positions base+i, base+64+i, ... so every load is one coalesced 256-byte
-// request, and all SUB loads of a task are issued before the first use.
+// request, and all SUB loads of a task are issued before the first use,
+// one task ahead of their use.
 // The source vertex of a position is found against a window of 64 vertex
 // boundaries held one per lane: src(p) = max{v : nindex[v] <= p}.  This
 // edge-centric split replaces the reference's per-vertex thread/warp split
@@ -140,18 +141,35 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
   const long long ngroups = ((long long)edges + TE - 1) / TE;
   const long long nwaves = (long long)gridDim.x * kInitWaves;
   long long emitted = 0;
-  for (long long g = (long long)blockIdx.x * kInitWaves + wid; g < ngroups; g += nwaves) {
-    const long long gbeg = g * TE;
-    int nl[SUB], w[SUB];
-    int base = gsrc[g];
+  // The stream loads of a wave's next task are issued before the current
+  // task is processed (after the current task's boundary window, so waiting
+  // for that window does not wait for them).
+  long long g = (long long)blockIdx.x * kInitWaves + wid;
+  int nl[SUB], w[SUB], base = 0;
+  if (g < ngroups) {
+    base = gsrc[g];
 #pragma unroll
     for (int u = 0; u < SUB; u++) {
-      const long long p = gbeg + u * kWave + lane;
-      nl[u] = p < edges ? nlist[p] : -1;
-      if (!HEAVY) w[u] = p < edges ? eweight[p] : 0;
+      const long long p = g * TE + u * kWave + lane;
+      nl[u] = p < edges ? ld_stream(&nlist[p]) : -1;
+      if (!HEAVY) w[u] = p < edges ? ld_stream(&eweight[p]) : 0;
     }
+  }
+  for (; g < ngroups; g += nwaves) {
+    const long long gbeg = g * TE;
     long long bi = (long long)base + 1 + lane;
     int bnd = bi <= nodes ? nindex[bi] : INT_MAX;
+    const long long gn = g + nwaves;
+    int nl_n[SUB], w_n[SUB], base_n = 0;
+    if (gn < ngroups) {
+      base_n = gsrc[gn];
+#pragma unroll
+      for (int u = 0; u < SUB; u++) {
+        const long long p = gn * TE + u * kWave + lane;
+        nl_n[u] = p < edges ? ld_stream(&nlist[p]) : -1;
+        if (!HEAVY) w_n[u] = p < edges ? ld_stream(&eweight[p]) : 0;
+      }
+    }
     int src[SUB];
 #pragma unroll
     for (int u = 0; u < SUB; u++) {
@@ -215,6 +233,12 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
         emitted += keep;
         stage_push(st, keep, make_int4(a[u], b[u], wt, (int)p), wl, cnt);
       }
+    }
+    base = base_n;
+#pragma unroll
+    for (int u = 0; u < SUB; u++) {
+      nl[u] = nl_n[u];
+      if (!HEAVY) w[u] = w_n[u];
     }
   }
   stage_flush(st, wl, cnt);

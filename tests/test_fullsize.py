@@ -1,12 +1,16 @@
 """Full-size configurations on the GPU, through the C-ABI.
 
-C2 RMAT-24 and C3 RGG-2^24: every filter mode against the oracle's serial
-Kruskal, bit for bit.  C5 RMAT-26 with weights in [1, 64] (heavy ties, a
-forest, ~2e9 directed edges): the forest is unique under the (w, j) order,
-so the unfiltered, plain and learned runs must give identical bytes; the
-oracle's union-find then checks that it is a spanning forest, and the
-weights agree with the OpenMP Boruvka restatement.  Set EMST_SKIP_RMAT26=1
+C1 grid 512x512, C2 RMAT-24 and C3 RGG-2^24: the solve's counts (records,
+rounds, worklist sizes entering and leaving the rounds, weight) against the
+oracle's round schedule committed in tests/golden/schedules.json (made by
+make_schedules.py); C2 and C3 also in every filter mode against the
+oracle's serial Kruskal, bit for bit.  C5 RMAT-26 with weights in [1, 64]
+(heavy ties, a forest, ~2e9 directed edges): the forest is unique under
+the (w, j) order, so the unfiltered, plain and learned runs must give
+identical bytes; the oracle's union-find then checks that it is a spanning
+forest, and the weights agree with the OpenMP Boruvka restatement.  Set EMST_SKIP_RMAT26=1
 to leave C5 out."""
+import json
 import os
 import time
 
@@ -15,10 +19,13 @@ import pytest
 
 import emst
 import oracle_lib as O
+from golden.make_schedules import digest
 
 pytestmark = [pytest.mark.gpu, pytest.mark.slow]
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 MODEL = os.path.join(ROOT, "mst-artifact-sc23_amd", "filter_mlp.f32")
+with open(os.path.join(ROOT, "tests", "golden", "schedules.json")) as _f:
+    SCHEDULES = json.load(_f)
 
 
 def _solver():
@@ -41,6 +48,29 @@ def _modes(s, g):
     return out
 
 
+def check_schedules(s, g, cfg):
+    """The solve's deterministic counts against the oracle's schedule."""
+    ent = SCHEDULES[cfg]
+    assert (g.nodes, g.edges) == (ent["nodes"], ent["edges"]) and digest(g) == ent["sha256"]
+    s.load(g)
+    for run in ent["runs"]:
+        st = s.solve(emst.FILTER_PLAIN if run["use_filter"] else emst.FILTER_OFF, run["threshold"])
+        got = (st.light, st.heavy, st.light_rounds, st.heavy_rounds, st.round_in_sum, st.round_out_sum,
+               emst.mst_weight(g, s.fetch()))
+        want = (run["L"], run["H"], run["light_rounds"], run["heavy_rounds"],
+                sum(run["light_in"]) + sum(run["heavy_in"]), sum(run["light_out"]) + sum(run["heavy_out"]),
+                run["weight"])
+        print(f"  schedule T={run['threshold']}: {got}", flush=True)
+        assert got == want, (cfg, run["threshold"])
+
+
+def test_c1_grid_schedule():
+    g = emst.gen_grid(512, 512, 1, 1, 1 << 20)
+    s = _solver()
+    check_schedules(s, g, "grid512")
+    s.close()
+
+
 @pytest.mark.parametrize("cfg", ["rmat24", "rgg24"])
 def test_c2_c3_against_kruskal(cfg):
     t0 = time.time()
@@ -49,6 +79,7 @@ def test_c2_c3_against_kruskal(cfg):
     print(f"{cfg}: n={g.nodes} m={g.edges} generated in {time.time()-t0:.1f}s", flush=True)
     s = _solver()
     res = _modes(s, g)
+    check_schedules(s, g, cfg)
     s.close()
     ref = O.kruskal(g)
     for name, (got, st) in res.items():

@@ -43,6 +43,9 @@ def check(solver, g, use_filter=None, T=None, rounds=True, kruskal=True):
     if rounds:
         assert st.light_rounds == sch.light_rounds
         assert st.heavy_rounds == (sch.heavy_rounds if use_filter else 0)
+        sin = sum(sch.light_w[i] for i in range(sch.light_rounds)) + sum(sch.heavy_w[i] for i in range(sch.heavy_rounds))
+        sout = sum(sch.light_s[i] for i in range(sch.light_rounds)) + sum(sch.heavy_s[i] for i in range(sch.heavy_rounds))
+        assert (st.round_in_sum, st.round_out_sum) == (sin, sout)
     assert st.mst_edges == sch.mst_edges
     return got, st
 

@@ -220,3 +220,21 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_schedule_fixture_grid512_recomputed():
+    """The committed schedule fixture of the 512x512 grid, recomputed by the
+    oracle from a freshly generated graph (the full-size fixtures are made
+    the same way by tests/golden/make_schedules.py)."""
+    from golden.make_schedules import CONFIGS, digest, schedule_entry
+    with open(os.path.join(os.path.dirname(__file__), "golden", "schedules.json")) as f:
+        ent = json.load(f)["grid512"]
+    gen, thresholds = CONFIGS["grid512"]
+    g = gen()
+    assert digest(g) == ent["sha256"]
+    for run, T in zip(ent["runs"], thresholds):
+        uf, t = emst.plain_threshold(g) if T is None else (1, T)
+        assert schedule_entry(g, uf, t) == run
+    # closed forms: a connected grid has n - 1 forest edges and one component
+    assert ent["runs"][0]["mst_edges"] == g.nodes - 1 and ent["runs"][0]["components"] == 1
+    assert O.schedule(g, 0, emst.INT_MAX)[1].light == g.edges // 2
