@@ -38,6 +38,10 @@ struct emst_ctx {
   long long* d_stats = nullptr;
   int* h_flag = nullptr;
   int* d_flag = nullptr;
+  long long* h_report = nullptr;      // host-mapped solve report (k_report)
+  long long* d_report = nullptr;
+  int* h_decide = nullptr;            // host-mapped learned filter decision
+  int* d_decide = nullptr;
   hipEvent_t ev[8] = {};
   int32_t last_nodes = 0;
   // learned threshold

@@ -50,6 +50,7 @@ constexpr int kInitWaves = 4;     // waves per workgroup in k_init_wl
 constexpr int kHflagSlots = 512;  // host-visible round results per solve
 constexpr int kSelItems = 4;      // records per thread per k_select tile
 constexpr int kTbl = 1024;        // LDS min-combining table entries
+constexpr int kReportWords = ST_COUNT + 3;  // counters, threshold, filter flag, quantile bits
 
 // ---------------------------------------------------------------------------
 // k_setup: initPM (ECL_MST.cu:147-154) plus the start vertex of every
@@ -544,6 +545,20 @@ __global__ __launch_bounds__(WAVES * kWave) void k_join(
   }
 }
 
+// Copies the solve's counters, threshold and quantile into host-mapped
+// memory with system-scope stores, so the host reads them after the final
+// event without further copies.
+__global__ __launch_bounds__(64) void k_report(const long long* __restrict__ stats, const int* __restrict__ thr,
+                                               const float* __restrict__ q, long long* __restrict__ out)
+{
+  const int t = threadIdx.x;
+  long long v = 0;
+  if (t < ST_COUNT) v = stats[t];
+  else if (t < ST_COUNT + 2) v = thr ? thr[t - ST_COUNT] : 0;
+  else if (t == ST_COUNT + 2) v = q ? (long long)__float_as_uint(q[0]) : 0;
+  if (t < kReportWords) __hip_atomic_store(&out[t], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Host-sync ablation: a plain kernel2 without the fused tail.
 __global__ __launch_bounds__(256) void k_join_plain(const int4* __restrict__ wl, int n, int* __restrict__ parent,
                                                     u64* __restrict__ minv, uint8_t* __restrict__ inMST)
@@ -739,9 +754,14 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
   const JoinFn join = join_fn(waves);
 
   if (!fixed) HIPCHK(hipEventRecord(ctx->ev[0], s));
-  if (learned)
+  if (learned) {
+    // features and inference decide the threshold on the device; the filter
+    // decision is also posted to host-mapped memory for the phase loop
+    __atomic_store_n(ctx->h_decide, -1, __ATOMIC_RELEASE);
     launch_learned_threshold(s, nodes, edges, d_nindex, d_eweight, ctx->d_mlp16, ctx->have_model, ctx->d_thr, ctx->d_q,
-                             ctx->d_feat, ctx->d_samples);
+                             ctx->d_feat, ctx->d_samples, ctx->d_decide);
+    HIPCHK(hipGetLastError());
+  }
   {
     const int blocks = (int)(((long long)nodes + 255) / 256);
     hipLaunchKernelGGL(k_setup, dim3(blocks), dim3(256), 0, s, nodes, d_nindex, ctx->d_parent, ctx->d_minv,
@@ -753,6 +773,21 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
   int init_launches = 0;
   int slot = 0;
   for (int phase = 0; phase < nphases; phase++) {
+    if (phase == 1 && learned) {
+      // The device decided the filter; an unfiltered decision needs no
+      // heavy phase, so none is enqueued.
+      int v;
+      while ((v = __atomic_load_n(ctx->h_decide, __ATOMIC_ACQUIRE)) < 0) {
+        const hipError_t q = hipStreamQuery(s);
+        if (q == hipSuccess) {
+          v = __atomic_load_n(ctx->h_decide, __ATOMIC_ACQUIRE);
+          break;
+        }
+        if (q != hipErrorNotReady) return hip_fail(ctx, q, "threshold inference");
+      }
+      if (v < 0) return ctx_fail(ctx, EMST_E_HIP, "threshold inference posted no decision");
+      if (v == 0) break;
+    }
     if (phase == 1) {
       const int blocks = (int)(((long long)nodes + 255) / 256);
       hipLaunchKernelGGL(k_pick_big, dim3(1), dim3(kWave), 0, s, nodes, ctx->d_parent, &d_cnt[2], dthr);
@@ -837,6 +872,11 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
     }
   }
   if (fixed) return EMST_OK;  // captured: the caller times and reads back
+  if (stats) {
+    hipLaunchKernelGGL(k_report, dim3(1), dim3(64), 0, s, d_stats, learned ? ctx->d_thr : nullptr,
+                       learned ? ctx->d_q : nullptr, ctx->d_report);
+    HIPCHK(hipGetLastError());
+  }
   HIPCHK(hipEventRecord(ctx->ev[1], s));
   HIPCHK(hipEventSynchronize(ctx->ev[1]));
   HIPCHK(hipGetLastError());
@@ -849,17 +889,14 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       HIPCHK(hipEventElapsedTime(&tph[ph], ctx->ev[2 + 2 * ph], ctx->ev[3 + 2 * ph]));
       init_ms += tph[ph];
     }
-    long long hs[ST_COUNT];
-    HIPCHK(hipMemcpy(hs, d_stats, sizeof(hs), hipMemcpyDeviceToHost));
+    long long hs[kReportWords];
+    for (int i = 0; i < kReportWords; i++) hs[i] = __atomic_load_n(&ctx->h_report[i], __ATOMIC_ACQUIRE);
     std::memset(stats, 0, sizeof(*stats));
     if (learned) {
-      int thr[2];
-      float q;
-      HIPCHK(hipMemcpy(thr, ctx->d_thr, 8, hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpy(&q, ctx->d_q, 4, hipMemcpyDeviceToHost));
-      stats->threshold_used = thr[0];
-      stats->filter_used = thr[1];
-      stats->learned_q = q;
+      stats->threshold_used = (int32_t)hs[ST_COUNT];
+      stats->filter_used = (int32_t)hs[ST_COUNT + 1];
+      const unsigned qb = (unsigned)hs[ST_COUNT + 2];
+      std::memcpy(&stats->learned_q, &qb, 4);
     } else {
       stats->threshold_used = T;
       stats->filter_used = nphases == 2 ? 1 : 0;
@@ -907,6 +944,10 @@ int emst_create(int device, emst_ctx** out)
   ok = ok && hipMalloc((void**)&ctx->d_stats, ST_COUNT * sizeof(long long)) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&ctx->h_flag, kHflagSlots * sizeof(int), hipHostMallocMapped) == hipSuccess;
   ok = ok && hipHostGetDevicePointer((void**)&ctx->d_flag, ctx->h_flag, 0) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&ctx->h_report, kReportWords * sizeof(long long), hipHostMallocMapped) == hipSuccess;
+  ok = ok && hipHostGetDevicePointer((void**)&ctx->d_report, ctx->h_report, 0) == hipSuccess;
+  ok = ok && hipHostMalloc((void**)&ctx->h_decide, sizeof(int), hipHostMallocMapped) == hipSuccess;
+  ok = ok && hipHostGetDevicePointer((void**)&ctx->d_decide, ctx->h_decide, 0) == hipSuccess;
   for (int i = 0; ok && i < 8; i++) ok = hipEventCreate(&ctx->ev[i]) == hipSuccess;
   ok = ok && hipMalloc((void**)&ctx->d_mlp16, kParams * 2) == hipSuccess &&
        hipMalloc((void**)&ctx->d_thr, 8) == hipSuccess && hipMalloc((void**)&ctx->d_q, 4) == hipSuccess &&
@@ -939,6 +980,8 @@ void emst_destroy(emst_ctx* ctx)
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
+  if (ctx->h_report) (void)hipHostFree(ctx->h_report);
+  if (ctx->h_decide) (void)hipHostFree(ctx->h_decide);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1104,7 +1147,7 @@ int emst_learned_threshold(emst_ctx* ctx, int* use_filter, int32_t* threshold, f
   if (!ctx->loaded) return ctx_fail(ctx, EMST_E_STATE, "emst_learned_threshold before emst_load");
   HIPCHK(hipSetDevice(ctx->device));
   launch_learned_threshold(ctx->stream, ctx->n, ctx->m, ctx->d_nindex, ctx->d_eweight, ctx->d_mlp16, ctx->have_model,
-                           ctx->d_thr, ctx->d_q, ctx->d_feat, ctx->d_samples);
+                           ctx->d_thr, ctx->d_q, ctx->d_feat, ctx->d_samples, nullptr);
   HIPCHK(hipGetLastError());
   int thr[2];
   float qq;

@@ -29,6 +29,20 @@
 
 namespace emst {
 
+constexpr int kWaveSize = 64;
+
+// Output unit: the kH-term dot product of the second hidden layer with W3,
+// one term per lane of the first wavefront, summed by a butterfly.
+// Lanes t >= kH return garbage; the caller uses lane 0.
+__device__ __forceinline__ float out_unit(const float* s_p, const float* s_h2, int t)
+{
+  static_assert(kH == kWaveSize, "one hidden unit per lane");
+  float y = t < kH ? s_h2[t] * s_p[kW3 + t] : 0.f;
+#pragma unroll
+  for (int off = kWaveSize / 2; off >= 1; off >>= 1) y += __shfl_xor(y, off);
+  return y + s_p[kB3];
+}
+
 __device__ __forceinline__ unsigned ecl_hash(unsigned v)  // ECL_MST.cu:390-395
 {
   v = ((v >> 16) ^ v) * 0x45d9f3bu;
@@ -56,33 +70,51 @@ __device__ void features_block(int nodes, int edges, const int* __restrict__ nin
   for (int i = t; i < kWeightSamples; i += blockDim.x)
     s_w[i] = edges > 0 ? eweight[ecl_hash((unsigned)i) % (unsigned)edges] : 0;
   __syncthreads();
-  for (int i = t; i < kVertexSamples; i += blockDim.x) {
-    const unsigned v = ecl_hash((unsigned)i ^ 0x5bd1e995u) % (unsigned)nodes;
-    const int deg = nindex[v + 1] - nindex[v];
+  // all of a thread's degree loads are issued before the first histogram update
+  constexpr int kPer = (kVertexSamples + kFeatThreads - 1) / kFeatThreads;
+  int deg[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    const int i = t + k * (int)blockDim.x;
+    deg[k] = -1;
+    if (i < kVertexSamples) {
+      const unsigned v = ecl_hash((unsigned)i ^ 0x5bd1e995u) % (unsigned)nodes;
+      deg[k] = nindex[v + 1] - nindex[v];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    if (deg[k] < 0) continue;
     int bin = 0;  // 0 = isolated
-    if (deg > 0) {
-      bin = 1 + (31 - __clz(deg));
+    if (deg[k] > 0) {
+      bin = 1 + (31 - __clz(deg[k]));
       if (bin > kDegBins) bin = kDegBins;
     }
     atomicAdd(&s_hist[bin], 1);
   }
   // bitonic sort of the weight samples (kWeightSamples is a power of two)
+  // Bitonic sort, one sample per thread: partners closer than a wavefront
+  // exchange through cross-lane reads, farther ones through LDS.
+  static_assert(kWeightSamples == kFeatThreads, "one weight sample per thread");
+  int v = s_w[t];
   for (int k = 2; k <= kWeightSamples; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
-      __syncthreads();
-      for (int i = t; i < kWeightSamples; i += blockDim.x) {
-        const int l = i ^ j;
-        if (l > i) {
-          const bool up = (i & k) == 0;
-          const int a = s_w[i], b = s_w[l];
-          if ((a > b) == up) {
-            s_w[i] = b;
-            s_w[l] = a;
-          }
-        }
+      int o;
+      if (j >= kWaveSize) {
+        __syncthreads();
+        s_w[t] = v;
+        __syncthreads();
+        o = s_w[t ^ j];
+      } else {
+        o = __shfl_xor(v, j);
       }
+      // ascending blocks keep the minimum at the lower index of a pair
+      const bool up = (t & k) == 0, lower = (t & j) == 0;
+      v = (lower == up) ? min(v, o) : max(v, o);
     }
   }
+  __syncthreads();
+  s_w[t] = v;
   __syncthreads();
   for (int i = t; i < kWeightSamples; i += blockDim.x) samples_sorted[i] = s_w[i];
   if (t == 0) {
@@ -152,7 +184,7 @@ __device__ void plain_threshold_dev(int nodes, int edges, const int* __restrict_
 __global__ __launch_bounds__(kFeatThreads) void k_learned_threshold(
     int nodes, int edges, const int* __restrict__ nindex, const int* __restrict__ eweight,
     const unsigned short* __restrict__ p16, int have_model, int* __restrict__ out, float* __restrict__ outf,
-    float* __restrict__ feat, int* __restrict__ samples)
+    float* __restrict__ feat, int* __restrict__ samples, int* __restrict__ host_decide)
 {
   __shared__ float s_p[kParams];
   __shared__ float s_feat[kIn];
@@ -173,14 +205,14 @@ __global__ __launch_bounds__(kFeatThreads) void k_learned_threshold(
     s_h2[t] = a > 0.f ? a : 0.f;
   }
   __syncthreads();
+  float y = 0.f;
+  if (t < kWaveSize) y = out_unit(s_p, s_h2, t);
   if (t == 0) {
     int use_filter = 0, T = INT_MAX;
     float q = -1.f;
     if (!have_model) {
       plain_threshold_dev(nodes, edges, eweight, &T, &use_filter);
     } else if (edges > 0) {
-      float y = s_p[kB3];
-      for (int k = 0; k < kH; k++) y = fmaf(s_h2[k], s_p[kW3 + k], y);
       if (!(y == y)) y = 0.f;  // NaN: unfiltered (exact either way)
       y = fminf(fmaxf(y, kMinLog2Q), 0.f);
       q = exp2f(y);
@@ -194,6 +226,8 @@ __global__ __launch_bounds__(kFeatThreads) void k_learned_threshold(
     out[0] = T;
     out[1] = use_filter;
     outf[0] = q;
+    // the host learns the filter decision without a copy (host-mapped)
+    if (host_decide) __hip_atomic_store(host_decide, use_filter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
@@ -231,13 +265,13 @@ __global__ __launch_bounds__(kFeatThreads) void k_learned_batch(
     s_h2[t] = a > 0.f ? a : 0.f;
   }
   __syncthreads();
+  float y = 0.f;
+  if (t < kWaveSize) y = out_unit(s_p, s_h2, t);
   if (t == 0) {
     int use_filter = 0, T = INT_MAX;
     if (!have_model) {
       plain_threshold_dev(nodes, edges, ew, &T, &use_filter);
     } else if (edges > 0) {
-      float y = s_p[kB3];
-      for (int k = 0; k < kH; k++) y = fmaf(s_h2[k], s_p[kW3 + k], y);
       if (!(y == y)) y = 0.f;
       y = fminf(fmaxf(y, kMinLog2Q), 0.f);
       const float q = exp2f(y);
@@ -496,10 +530,10 @@ int emst_mlp_train(int device, int32_t batch, const float* X, const float* T, in
 namespace emst {
 void launch_learned_threshold(hipStream_t s, int nodes, int edges, const int* nindex, const int* eweight,
                               const unsigned short* p16, int have_model, int* out, float* outf, float* feat,
-                              int* samples)
+                              int* samples, int* host_decide)
 {
   hipLaunchKernelGGL(k_learned_threshold, dim3(1), dim3(kFeatThreads), 0, s, nodes, edges, nindex, eweight, p16,
-                     have_model, out, outf, feat, samples);
+                     have_model, out, outf, feat, samples, host_decide);
 }
 void launch_learned_batch(hipStream_t s, int graphs, const int* nindex, const int* eweight, const long long* voff,
                           const long long* eoff, const unsigned short* p16, int have_model, int* gthr, int* gflag)

@@ -24,7 +24,7 @@ constexpr float kNoFilterQ = 0.9f;      // q at or above: unfiltered path
 
 void launch_learned_threshold(hipStream_t s, int nodes, int edges, const int* nindex, const int* eweight,
                               const unsigned short* p16, int have_model, int* out, float* outf, float* feat,
-                              int* samples);
+                              int* samples, int* host_decide);
 
 void launch_learned_batch(hipStream_t s, int graphs, const int* nindex, const int* eweight, const long long* voff,
                           const long long* eoff, const unsigned short* p16, int have_model, int* gthr, int* gflag);
