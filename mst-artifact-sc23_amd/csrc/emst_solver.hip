@@ -656,7 +656,7 @@ emst_config default_config()
   c.stage_recs = 512;
   c.tail_max = 4096;
   c.flags = 0;
-  c.init_grid_per_cu = 8;
+  c.init_grid_per_cu = 4;
   return c;
 }
 
@@ -1208,7 +1208,7 @@ int emst_autotune(emst_ctx* ctx, int use_filter, int32_t threshold, int32_t reps
   };
   const std::vector<Knob> knobs = {
       {&emst_config::tile_edges, {256, 512, 1024}},   {&emst_config::stage_recs, {128, 256, 512, 1024}},
-      {&emst_config::init_grid_per_cu, {4, 8, 16}},    {&emst_config::block, {256, 512, 1024}},
+      {&emst_config::init_grid_per_cu, {3, 4, 6, 8}},    {&emst_config::block, {256, 512, 1024}},
       {&emst_config::grid_per_cu, {1, 2, 4, 8}},       {&emst_config::tail_max, {-1, 1024, 4096, 16384}}};
   for (int pass = 0; pass < 2; pass++) {
     for (const Knob& k : knobs) {
