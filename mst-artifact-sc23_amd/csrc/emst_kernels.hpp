@@ -14,6 +14,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <climits>
 
 namespace emst {
 
@@ -148,6 +149,21 @@ __device__ __forceinline__ void stage_push(Stage& s, bool keep, int4 rec, int4* 
   if (s.n + k > s.cap) stage_flush(s, out, cnt);
   if (keep) s.buf[s.n + lanes_below(mask)] = rec;
   s.n += k;
+}
+
+// Inclusive maximum-scan across the 64 lanes of a wavefront with DPP moves
+// (ALU-speed, no LDS crossbar): doubling shifts inside each row of 16
+// lanes, then the last lane of rows 0 and 2 into rows 1 and 3, then lane 31
+// into rows 2 and 3.  Lanes without a source take INT_MIN, the identity.
+__device__ __forceinline__ int wave_max_scan(int x)
+{
+  x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x111, 0xf, 0xf, false));  // row_shr:1
+  x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x112, 0xf, 0xf, false));  // row_shr:2
+  x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x114, 0xf, 0xf, false));  // row_shr:4
+  x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x118, 0xf, 0xf, false));  // row_shr:8
+  x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  x = max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return x;
 }
 
 // Number of lanes whose bnd is <= p, for bnd sorted ascending across the

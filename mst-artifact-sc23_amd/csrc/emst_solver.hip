@@ -47,6 +47,7 @@ using namespace emst;
 namespace {
 
 constexpr int kInitWaves = 4;     // waves per workgroup in k_init_wl
+constexpr int kMarkSpan = 256;   // longest vertex range of a task resolved by marks
 constexpr int kHflagSlots = 512;  // host-visible round results per solve
 constexpr int kSelItems = 4;      // records per thread per k_select tile
 constexpr int kTbl = 1024;        // LDS min-combining table entries
@@ -113,8 +114,11 @@ __global__ __launch_bounds__(256) void k_zero(uint8_t* __restrict__ p, long long
 // positions base+i, base+64+i, ... so every load is one coalesced 256-byte
 // request, and all SUB loads of a task are issued before the first use,
 // one task ahead of their use.
-// The source vertex of a position is found against a window of 64 vertex
-// boundaries held one per lane: src(p) = max{v : nindex[v] <= p}.  This
+// The source vertex of a position, src(p) = max{v : nindex[v] <= p}, comes
+// from marks: every vertex whose adjacency starts inside the task writes
+// its id at that offset in a per-wave LDS array, and a DPP max-scan over the
+// offsets fills the rest.  Tasks whose vertex range exceeds kMarkSpan (runs
+// of zero-degree vertices) search windows of 64 boundaries instead.  This
 // edge-centric split replaces the reference's per-vertex thread/warp split
 // at degree 4 (ECL_MST.cu:167, 185): every wave streams whole 256-byte
 // lines whatever the degrees, and hub vertices spread over many waves.
@@ -139,6 +143,7 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
   const int lane = lane_id();
   const int wid = threadIdx.x / kWave;
   Stage st{lds_stage + wid * stage_cap, stage_cap, 0};
+  int* marks = reinterpret_cast<int*>(lds_stage + kInitWaves * stage_cap) + wid * TE;
   const long long ngroups = ((long long)edges + TE - 1) / TE;
   const long long nwaves = (long long)gridDim.x * kInitWaves;
   long long emitted = 0;
@@ -146,9 +151,10 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
   // task is processed (after the current task's boundary window, so waiting
   // for that window does not wait for them).
   long long g = (long long)blockIdx.x * kInitWaves + wid;
-  int nl[SUB], w[SUB], base = 0;
+  int nl[SUB], w[SUB], base = 0, vend = 0;
   if (g < ngroups) {
     base = gsrc[g];
+    vend = g + 1 < ngroups ? gsrc[g + 1] : nodes - 1;
 #pragma unroll
     for (int u = 0; u < SUB; u++) {
       const long long p = g * TE + u * kWave + lane;
@@ -158,12 +164,25 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
   }
   for (; g < ngroups; g += nwaves) {
     const long long gbeg = g * TE;
+    const bool by_marks = vend - base <= kMarkSpan;
+    // vertex boundaries of this task (independent loads, issued first)
+    int stv[kMarkSpan / kWave];
     long long bi = (long long)base + 1 + lane;
-    int bnd = bi <= nodes ? nindex[bi] : INT_MAX;
+    int bnd = 0;
+    if (by_marks) {
+#pragma unroll
+      for (int q = 0; q < kMarkSpan / kWave; q++) {
+        const long long v = (long long)base + 1 + q * kWave + lane;
+        stv[q] = (q * kWave < vend - base && v <= vend) ? nindex[v] : INT_MAX;
+      }
+    } else {
+      bnd = bi <= nodes ? nindex[bi] : INT_MAX;
+    }
     const long long gn = g + nwaves;
-    int nl_n[SUB], w_n[SUB], base_n = 0;
+    int nl_n[SUB], w_n[SUB], base_n = 0, vend_n = 0;
     if (gn < ngroups) {
       base_n = gsrc[gn];
+      vend_n = gn + 1 < ngroups ? gsrc[gn + 1] : nodes - 1;
 #pragma unroll
       for (int u = 0; u < SUB; u++) {
         const long long p = gn * TE + u * kWave + lane;
@@ -171,26 +190,55 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
         if (!HEAVY) w_n[u] = p < edges ? ld_stream(&eweight[p]) : 0;
       }
     }
+    // Source vertex of every position: src(p) = max{v : nindex[v] <= p}.
     int src[SUB];
+    if (by_marks) {
+      // Every vertex v in (base, vend] whose adjacency starts inside the
+      // task writes v at its start offset (the largest v wins where
+      // zero-degree vertices share a start); a prefix maximum over the
+      // task's offsets then gives the source of every position.
 #pragma unroll
-    for (int u = 0; u < SUB; u++) {
-      const long long pl = gbeg + u * kWave + lane;
-      const int p = pl < edges ? (int)pl : INT_MAX - 1;
-      bool done = pl >= edges;
-      int s = 0;
-      for (;;) {
-        const int c = count_le(bnd, p);
-        const int b63 = __shfl(bnd, kWave - 1);
-        if (!done && p < b63) {
-          s = base + c;
-          done = true;
-        }
-        if (__all(done)) break;
-        base += kWave;
-        bi = (long long)base + 1 + lane;
-        bnd = bi <= nodes ? nindex[bi] : INT_MAX;
+      for (int u = 0; u < SUB; u++) marks[u * kWave + lane] = -1;
+      wave_lds_sync();
+#pragma unroll
+      for (int q = 0; q < kMarkSpan / kWave; q++) {
+        const long long off = (long long)stv[q] - gbeg;
+        if (off > 0 && off < TE) atomicMax(&marks[off], base + 1 + q * kWave + lane);
       }
-      src[u] = s;
+      wave_lds_sync();
+      // prefix maximum in position order: a wave max-scan per row of 64
+      // positions, carried from row to row
+      int carry = base;
+#pragma unroll
+      for (int u = 0; u < SUB; u++) {
+        const int x = max(wave_max_scan(marks[u * kWave + lane]), carry);
+        src[u] = x;
+        carry = __builtin_amdgcn_readlane(x, kWave - 1);
+      }
+      wave_lds_sync();  // the next task rewrites the marks
+    } else {
+      // long vertex ranges (runs of zero-degree vertices): search windows
+      // of 64 vertex boundaries held one per lane
+#pragma unroll
+      for (int u = 0; u < SUB; u++) {
+        const long long pl = gbeg + u * kWave + lane;
+        const int p = pl < edges ? (int)pl : INT_MAX - 1;
+        bool done = pl >= edges;
+        int sv = 0;
+        for (;;) {
+          const int c = count_le(bnd, p);
+          const int b63 = __shfl(bnd, kWave - 1);
+          if (!done && p < b63) {
+            sv = base + c;
+            done = true;
+          }
+          if (__all(done)) break;
+          base += kWave;
+          bi = (long long)base + 1 + lane;
+          bnd = bi <= nodes ? nindex[bi] : INT_MAX;
+        }
+        src[u] = sv;
+      }
     }
     if (!HEAVY) {
 #pragma unroll
@@ -236,6 +284,7 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
       }
     }
     base = base_n;
+    vend = vend_n;
 #pragma unroll
     for (int u = 0; u < SUB; u++) {
       nl[u] = nl_n[u];
@@ -741,7 +790,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
   const int round_grid = std::max(1, ctx->cus * std::max(1, cfg.grid_per_cu));
   const int init_grid = std::max(1, ctx->cus * std::max(1, cfg.init_grid_per_cu));
   const int stage_cap = std::max(64, cfg.stage_recs);
-  const size_t init_lds = (size_t)kInitWaves * stage_cap * sizeof(int4);
+  const size_t init_lds = (size_t)kInitWaves * stage_cap * sizeof(int4) + (size_t)kInitWaves * task_edges * sizeof(int);
   if (init_lds > ctx->lds_per_block) return ctx_fail(ctx, EMST_E_ARG, "stage_recs exceeds the LDS of a workgroup");
   const int nphases = (vt ? vt->any_filter : use_filter != EMST_FILTER_OFF) ? 2 : 1;
   const int* vgid = vt ? vt->vgid : nullptr;
@@ -1007,7 +1056,8 @@ int emst_set_config(emst_ctx* ctx, const emst_config* cfg)
   c.flags = cfg->flags;
   if (c.grid_per_cu < 1 || c.grid_per_cu > 64 || c.init_grid_per_cu < 1 || c.init_grid_per_cu > 64)
     return ctx_fail(ctx, EMST_E_ARG, "grid_per_cu out of range");
-  if (c.stage_recs < 64 || (size_t)kInitWaves * c.stage_recs * sizeof(int4) > ctx->lds_per_block)
+  if (c.stage_recs < 64 ||
+      (size_t)kInitWaves * (c.stage_recs * sizeof(int4) + c.tile_edges * sizeof(int)) > ctx->lds_per_block)
     return ctx_fail(ctx, EMST_E_ARG, "stage_recs out of range");
   if (c.tail_max > (1 << 20)) return ctx_fail(ctx, EMST_E_ARG, "tail_max out of range");
   if (c.tile_edges != 256 && c.tile_edges != 512 && c.tile_edges != 1024)

@@ -97,6 +97,34 @@ def test_rgg_and_er(solver):
     check(solver, emst.gen_er(20000, 10.0, 4, 1, 1 << 20))
 
 
+def test_zero_degree_runs(solver):
+    """Initialisation tasks over short and long vertex ranges: a dense block
+    (a task spans few vertices), then members spread among long runs of
+    zero-degree vertices (a task spans thousands), and trailing isolated
+    vertices; both source-vertex searches of the init kernels run."""
+    rng = np.random.default_rng(11)
+    n = 60000
+    dense = list(range(0, 300))
+    sparse = sorted(rng.choice(np.arange(300, n - 5000), 700, replace=False).tolist())
+    edges = []
+    for _ in range(6000):
+        u, v = rng.choice(dense, 2)
+        if u != v:
+            edges.append((int(u), int(v), int(rng.integers(1, 1000))))
+    for _ in range(4000):
+        u, v = rng.choice(sparse, 2)
+        if u != v:
+            edges.append((int(u), int(v), int(rng.integers(1, 1000))))
+    for i in range(0, len(sparse), 7):   # a few links between the regions
+        edges.append((int(rng.choice(dense)), int(sparse[i]), int(rng.integers(1, 1000))))
+    g = G(*csr_from_edges(n, edges))
+    for tile in (256, 512, 1024):
+        solver.set_config(tile_edges=tile)
+        check(solver, g)
+        check(solver, g, False, emst.INT_MAX)
+    solver.set_config()
+
+
 def test_negative_weights_follow_gpu_order(solver):
     # The reference GPU orders keys as unsigned (ECL_MST.cu:221), so a
     # negative weight sorts above every non-negative one there; the
