@@ -31,6 +31,7 @@ struct emst_ctx {
   unsigned* d_vclass = nullptr;
   size_t cap_nodes = 0;
   int4* d_wl[2] = {nullptr, nullptr};
+  emst::u64* d_cand = nullptr;        // first-round candidate bitmasks
   size_t cap_wl = 0;
   int* d_gsrc = nullptr;
   size_t cap_gsrc = 0;

@@ -112,13 +112,16 @@ __device__ __forceinline__ void hook(int a, int b, int* parent)
 // Guarded 64-bit atomic min (ECL_MST.cu:222-223): skip the atomic when a
 // relaxed device-scope read already shows a key no larger.  Within a round
 // minv only decreases, so a stale read can only cost an extra atomic.
-__device__ __forceinline__ void min_key(u64* minv, int rep, u64 key, bool guard)
+// Returns false when the key cannot be the minimum (a key no larger was
+// already there); true otherwise.
+__device__ __forceinline__ bool min_key(u64* minv, int rep, u64 key, bool guard)
 {
   if (guard) {
     const u64 cur = __hip_atomic_load(&minv[rep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (cur <= key) return;
+    if (cur <= key) return false;
   }
   atomicMin(&minv[rep], key);
+  return true;
 }
 
 // ---- per-wave LDS staging of worklist appends ------------------------------

@@ -357,7 +357,10 @@ __global__ __launch_bounds__(256) void k_flatten(int nodes, int* __restrict__ pa
 
 // Offers key to rep: min-combined in the workgroup's LDS table when the rep
 // holds (or claims) a slot within 4 probes, else a guarded global atomicMin.
-__device__ __forceinline__ void offer_key(int* tags, u64* keys, u64* minv, int rep, u64 key, bool guard,
+// Returns whether the offer can still be rep's minimum: the workgroup
+// table held no smaller key, or the guarded global read showed none.  The
+// winner of every rep is among these candidates (keys are distinct).
+__device__ __forceinline__ bool offer_key(int* tags, u64* keys, u64* minv, int rep, u64 key, bool guard,
                                           bool combine)
 {
   if (combine) {
@@ -371,12 +374,16 @@ __device__ __forceinline__ void offer_key(int* tags, u64* keys, u64* minv, int r
         if (t == -1) t = rep;
       }
       if (t == rep) {
-        atomicMin(&keys[sl], key);
-        return;
+        // a read before the atomic decides candidacy: a stale (larger)
+        // value only makes a record a candidate it need not be, and the
+        // non-returning LDS atomic keeps hot slots fast
+        const bool c = keys[sl] > key;
+        if (c) atomicMin(&keys[sl], key);
+        return c;
       }
     }
   }
-  min_key(minv, rep, key, guard);
+  return min_key(minv, rep, key, guard);
 }
 
 // ---------------------------------------------------------------------------
@@ -400,10 +407,11 @@ template <bool FIRST, int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void k_select(
     const int4* __restrict__ wl_in, const int* __restrict__ cnt_in, int4* __restrict__ wl_out,
     int* __restrict__ cnt_out, const int* __restrict__ parent, u64* __restrict__ minv, int guard, int combine,
-    long long* __restrict__ stats, int rounds_slot)
+    long long* __restrict__ stats, int rounds_slot, u64* __restrict__ cand)
 {
   constexpr int BT = WAVES * kWave;
   constexpr int TILE = BT * kSelItems;
+  const long long cand_words = ((long long)*cnt_in + kWave - 1) / kWave;
   __shared__ int s_tags[kTbl];
   __shared__ u64 s_keys[kTbl];
   __shared__ int s_wtot[WAVES];
@@ -438,16 +446,27 @@ __global__ __launch_bounds__(WAVES * kWave) void k_select(
     for (int k = 0; k < kSelItems; k++) {
       const long long i = base + k * BT + tid;
       keep[k] = false;
+      bool cx = false, cy = false;
       if (i < n_in) {
         const int a = FIRST ? r[k].x : find_root(r[k].x, parent);
         const int c = FIRST ? r[k].y : find_root(r[k].y, parent);
         if (a != c) {
           keep[k] = true;
+          const u64 key = make_key(r[k].z, r[k].w);
+          cx = offer_key(s_tags, s_keys, minv, a, key, guard, combine);
+          cy = offer_key(s_tags, s_keys, minv, c, key, guard, combine);
           r[k].x = a;
           r[k].y = c;
-          const u64 key = make_key(r[k].z, r[k].w);
-          offer_key(s_tags, s_keys, minv, a, key, guard, combine);
-          offer_key(s_tags, s_keys, minv, c, key, guard, combine);
+        }
+      }
+      if (FIRST) {
+        // the list is joined in place: candidate bits go to two bitmasks,
+        // one 64-bit word per wavefront row of records
+        const u64 mx = __ballot(cx), my = __ballot(cy);
+        const long long word = (base + k * BT) / kWave + wid;
+        if (lane == 0 && word < cand_words) {
+          cand[word] = mx;
+          cand[cand_words + word] = my;
         }
       }
       mine += keep[k];
@@ -499,14 +518,15 @@ __device__ void tail_rounds(int4* cur, int n, int4* nxt, int* parent, u64* minv,
     // join (kernel2 + kernel3)
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
       const int4 r = cur[i];
+      const int x = r.x, y = r.y;
       const u64 key = make_key(r.z, r.w);
-      const u64 kx = __hip_atomic_load(&minv[r.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const u64 ky = __hip_atomic_load(&minv[r.y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const u64 kx = __hip_atomic_load(&minv[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const u64 ky = __hip_atomic_load(&minv[y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const bool wx = key == kx, wy = key == ky;
-      if (wx) __hip_atomic_store(&minv[r.x], kNoKey, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (wy) __hip_atomic_store(&minv[r.y], kNoKey, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (wx) __hip_atomic_store(&minv[x], kNoKey, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (wy) __hip_atomic_store(&minv[y], kNoKey, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (wx || wy) {
-        hook(r.x, r.y, parent);
+        hook(x, y, parent);
         inMST[r.w] = 1;
       }
     }
@@ -557,7 +577,7 @@ template <int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void k_join(
     int4* __restrict__ wl, int* __restrict__ cnt, int4* __restrict__ wl_other, int* __restrict__ cnt_other,
     int* __restrict__ parent, u64* __restrict__ minv, uint8_t* __restrict__ inMST, int tail_max,
-    int* __restrict__ hflag, long long* __restrict__ stats, int rounds_slot)
+    int* __restrict__ hflag, long long* __restrict__ stats, int rounds_slot, const u64* __restrict__ cand)
 {
   const int n = *cnt;
   if (n > 0 && n <= tail_max) {
@@ -579,16 +599,25 @@ __global__ __launch_bounds__(WAVES * kWave) void k_join(
     __hip_atomic_store(hflag, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
   const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long cand_words = ((long long)n + kWave - 1) / kWave;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    // after the offer-only first round, only a candidate endpoint can have
+    // this record as its minimum; later rounds check both endpoints
+    bool cx = true, cy = true;
+    if (cand) {
+      cx = (cand[i / kWave] >> (i % kWave)) & 1ull;
+      cy = (cand[cand_words + i / kWave] >> (i % kWave)) & 1ull;
+      if (!cx && !cy) continue;
+    }
     const int4 r = wl[i];
+    const int x = r.x, y = r.y;
     const u64 key = make_key(r.z, r.w);
-    const u64 kx = minv[r.x];
-    const u64 ky = minv[r.y];
-    const bool wx = key == kx, wy = key == ky;
-    if (wx) minv[r.x] = kNoKey;
-    if (wy) minv[r.y] = kNoKey;
+    const bool wx = cx && minv[x] == key;
+    const bool wy = cy && minv[y] == key;
+    if (wx) minv[x] = kNoKey;
+    if (wy) minv[y] = kNoKey;
     if (wx || wy) {
-      hook(r.x, r.y, parent);
+      hook(x, y, parent);
       inMST[r.w] = 1;
     }
   }
@@ -615,14 +644,15 @@ __global__ __launch_bounds__(256) void k_join_plain(const int4* __restrict__ wl,
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     const int4 r = wl[i];
+    const int x = r.x, y = r.y;
     const u64 key = make_key(r.z, r.w);
-    const u64 kx = minv[r.x];
-    const u64 ky = minv[r.y];
+    const u64 kx = minv[x];
+    const u64 ky = minv[y];
     const bool wx = key == kx, wy = key == ky;
-    if (wx) minv[r.x] = kNoKey;
-    if (wy) minv[r.y] = kNoKey;
+    if (wx) minv[x] = kNoKey;
+    if (wy) minv[y] = kNoKey;
     if (wx || wy) {
-      hook(r.x, r.y, parent);
+      hook(x, y, parent);
       inMST[r.w] = 1;
     }
   }
@@ -632,8 +662,8 @@ __global__ __launch_bounds__(256) void k_join_plain(const int4* __restrict__ wl,
 typedef void (*InitFn)(const int*, const int*, const int*, const int*, int, int, int, const int*,
                        const unsigned*, int4*, int*, int, long long*, const int*, const int*, const int*,
                        const int*);
-typedef void (*SelFn)(const int4*, const int*, int4*, int*, const int*, u64*, int, int, long long*, int);
-typedef void (*JoinFn)(int4*, int*, int4*, int*, int*, u64*, uint8_t*, int, int*, long long*, int);
+typedef void (*SelFn)(const int4*, const int*, int4*, int*, const int*, u64*, int, int, long long*, int, u64*);
+typedef void (*JoinFn)(int4*, int*, int4*, int*, int*, u64*, uint8_t*, int, int*, long long*, int, const u64*);
 
 InitFn init_fn(bool heavy, int sub)
 {
@@ -753,6 +783,11 @@ int ensure_buffers(emst_ctx* ctx, int32_t nodes, int32_t edges, long long wl_cap
         std::snprintf(buf, sizeof(buf), "hipMalloc of a %zu-record worklist failed", want);
         return ctx_fail(ctx, EMST_E_NOMEM, buf);
       }
+    if (ctx->d_cand) (void)hipFree(ctx->d_cand);
+    ctx->d_cand = nullptr;
+    // two candidate bitmasks of the first round, one bit per record
+    if (hipMalloc((void**)&ctx->d_cand, 2 * ((want + kWave - 1) / kWave) * sizeof(u64)) != hipSuccess)
+      return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of the candidate bitmasks failed");
     ctx->cap_wl = want;
     ctx->buf_gen++;
   }
@@ -864,7 +899,8 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       if (size > 0) {
         const int blocks = (int)std::min<long long>(((long long)size + tile - 1) / tile, round_grid);
         hipLaunchKernelGGL(sel_first, dim3(blocks), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[cur ^ 1],
-                           &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot);
+                           &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot,
+                           ctx->d_cand);
       }
       while (size > 0) {
         hipLaunchKernelGGL(k_join_plain, dim3((size + 255) / 256), dim3(256), 0, s, wl[cur], size, ctx->d_parent,
@@ -872,7 +908,8 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
         HIPCHK(hipMemsetAsync(&d_cnt[cur ^ 1], 0, 4, s));
         const int blocks = (int)std::min<long long>(((long long)size + tile - 1) / tile, round_grid);
         hipLaunchKernelGGL(sel, dim3(blocks), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[cur ^ 1],
-                           &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot);
+                           &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot,
+                           (u64*)nullptr);
         cur ^= 1;
         HIPCHK(hipMemcpyAsync(&size, &d_cnt[cur], 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -891,11 +928,13 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       ctx->h_flag[slot] = -1;
       if (r == 0)
         hipLaunchKernelGGL(sel_first, dim3(round_grid), dim3(waves * kWave), 0, s, wl[0], &d_cnt[0], wl[1], &d_cnt[1],
-                           ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot);
+                           ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot, ctx->d_cand);
+      // the first join reads the candidate bitmasks, later ones the record bits
       hipLaunchKernelGGL(join, dim3(round_grid), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt], &d_cnt[nxt],
-                         ctx->d_parent, ctx->d_minv, d_inMST, tail_max, ctx->d_flag + slot, d_stats, rounds_slot);
+                         ctx->d_parent, ctx->d_minv, d_inMST, tail_max, ctx->d_flag + slot, d_stats, rounds_slot,
+                         (const u64*)(r == 0 ? ctx->d_cand : nullptr));
       hipLaunchKernelGGL(sel, dim3(round_grid), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt], &d_cnt[nxt],
-                         ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot);
+                         ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot, (u64*)nullptr);
       HIPCHK(hipGetLastError());
       slot++;
       if (fixed) {
@@ -1022,7 +1061,7 @@ void emst_destroy(emst_ctx* ctx)
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  void* ptrs[] = {ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, ctx->d_inMST, ctx->d_parent,
+  void* ptrs[] = {ctx->d_cand,   ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, ctx->d_inMST, ctx->d_parent,
                   ctx->d_minv,   ctx->d_wl[0], ctx->d_wl[1],   ctx->d_gsrc,  ctx->d_cnt,
                   ctx->d_stats,  ctx->d_vclass, ctx->d_mlp16,  ctx->d_thr,   ctx->d_q,
                   ctx->d_feat,   ctx->d_samples};
