@@ -357,9 +357,11 @@ __global__ __launch_bounds__(256) void k_flatten(int nodes, int* __restrict__ pa
 
 // Offers key to rep: min-combined in the workgroup's LDS table when the rep
 // holds (or claims) a slot within 4 probes, else a guarded global atomicMin.
-// Returns whether the offer can still be rep's minimum: the workgroup
-// table held no smaller key, or the guarded global read showed none.  The
-// winner of every rep is among these candidates (keys are distinct).
+// With CAND, returns whether the offer can still be rep's minimum: the
+// workgroup table held no smaller key, or the guarded global read showed
+// none.  The winner of every rep is among these candidates (keys are
+// distinct).
+template <bool CAND>
 __device__ __forceinline__ bool offer_key(int* tags, u64* keys, u64* minv, int rep, u64 key, bool guard,
                                           bool combine)
 {
@@ -374,6 +376,10 @@ __device__ __forceinline__ bool offer_key(int* tags, u64* keys, u64* minv, int r
         if (t == -1) t = rep;
       }
       if (t == rep) {
+        if (!CAND) {
+          atomicMin(&keys[sl], key);
+          return true;
+        }
         // a read before the atomic decides candidacy: a stale (larger)
         // value only makes a record a candidate it need not be, and the
         // non-returning LDS atomic keeps hot slots fast
@@ -407,7 +413,7 @@ template <bool FIRST, int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void k_select(
     const int4* __restrict__ wl_in, const int* __restrict__ cnt_in, int4* __restrict__ wl_out,
     int* __restrict__ cnt_out, const int* __restrict__ parent, u64* __restrict__ minv, int guard, int combine,
-    long long* __restrict__ stats, int rounds_slot, u64* __restrict__ cand)
+    long long* __restrict__ stats, int rounds_slot, u64* __restrict__ cand, const u64* __restrict__ dead)
 {
   constexpr int BT = WAVES * kWave;
   constexpr int TILE = BT * kSelItems;
@@ -435,10 +441,13 @@ __global__ __launch_bounds__(WAVES * kWave) void k_select(
   for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const long long base = t * TILE;
     int4 r[kSelItems];
+    bool live[kSelItems];
 #pragma unroll
     for (int k = 0; k < kSelItems; k++) {
       const long long i = base + k * BT + tid;
-      r[k] = i < n_in ? wl_in[i] : make_int4(0, 0, 0, 0);
+      // records joined by the first round are known to lie inside one tree
+      live[k] = i < n_in && !(dead && ((dead[(base + k * BT) / kWave + wid] >> lane) & 1ull));
+      r[k] = live[k] ? wl_in[i] : make_int4(0, 0, 0, 0);
     }
     bool keep[kSelItems];
     int mine = 0;
@@ -447,14 +456,14 @@ __global__ __launch_bounds__(WAVES * kWave) void k_select(
       const long long i = base + k * BT + tid;
       keep[k] = false;
       bool cx = false, cy = false;
-      if (i < n_in) {
+      if (live[k]) {
         const int a = FIRST ? r[k].x : find_root(r[k].x, parent);
         const int c = FIRST ? r[k].y : find_root(r[k].y, parent);
         if (a != c) {
           keep[k] = true;
           const u64 key = make_key(r[k].z, r[k].w);
-          cx = offer_key(s_tags, s_keys, minv, a, key, guard, combine);
-          cy = offer_key(s_tags, s_keys, minv, c, key, guard, combine);
+          cx = offer_key<FIRST>(s_tags, s_keys, minv, a, key, guard, combine);
+          cy = offer_key<FIRST>(s_tags, s_keys, minv, c, key, guard, combine);
           r[k].x = a;
           r[k].y = c;
         }
@@ -577,7 +586,7 @@ template <int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void k_join(
     int4* __restrict__ wl, int* __restrict__ cnt, int4* __restrict__ wl_other, int* __restrict__ cnt_other,
     int* __restrict__ parent, u64* __restrict__ minv, uint8_t* __restrict__ inMST, int tail_max,
-    int* __restrict__ hflag, long long* __restrict__ stats, int rounds_slot, const u64* __restrict__ cand)
+    int* __restrict__ hflag, long long* __restrict__ stats, int rounds_slot, u64* __restrict__ cand)
 {
   const int n = *cnt;
   if (n > 0 && n <= tail_max) {
@@ -607,18 +616,29 @@ __global__ __launch_bounds__(WAVES * kWave) void k_join(
     if (cand) {
       cx = (cand[i / kWave] >> (i % kWave)) & 1ull;
       cy = (cand[cand_words + i / kWave] >> (i % kWave)) & 1ull;
-      if (!cx && !cy) continue;
     }
-    const int4 r = wl[i];
-    const int x = r.x, y = r.y;
-    const u64 key = make_key(r.z, r.w);
-    const bool wx = cx && minv[x] == key;
-    const bool wy = cy && minv[y] == key;
-    if (wx) minv[x] = kNoKey;
-    if (wy) minv[y] = kNoKey;
-    if (wx || wy) {
-      hook(x, y, parent);
-      inMST[r.w] = 1;
+    bool joined = false;
+    if (cx || cy) {
+      const int4 r = wl[i];
+      const int x = r.x, y = r.y;
+      const u64 key = make_key(r.z, r.w);
+      const bool wx = cx && minv[x] == key;
+      const bool wy = cy && minv[y] == key;
+      if (wx) minv[x] = kNoKey;
+      if (wy) minv[y] = kNoKey;
+      if (wx || wy) {
+        hook(x, y, parent);
+        inMST[r.w] = 1;
+        joined = true;
+      }
+    }
+    if (cand) {
+      // a joined record lies inside one tree from now on: its bit in the
+      // first mask (read above by this wavefront only) now tells the next
+      // select to drop it without finds; lane 0 holds the row's first
+      // record, so it is active whenever any lane is
+      const u64 jm = __ballot(joined);
+      if (lane_id() == 0) cand[i / kWave] = jm;
     }
   }
 }
@@ -662,8 +682,9 @@ __global__ __launch_bounds__(256) void k_join_plain(const int4* __restrict__ wl,
 typedef void (*InitFn)(const int*, const int*, const int*, const int*, int, int, int, const int*,
                        const unsigned*, int4*, int*, int, long long*, const int*, const int*, const int*,
                        const int*);
-typedef void (*SelFn)(const int4*, const int*, int4*, int*, const int*, u64*, int, int, long long*, int, u64*);
-typedef void (*JoinFn)(int4*, int*, int4*, int*, int*, u64*, uint8_t*, int, int*, long long*, int, const u64*);
+typedef void (*SelFn)(const int4*, const int*, int4*, int*, const int*, u64*, int, int, long long*, int, u64*,
+                      const u64*);
+typedef void (*JoinFn)(int4*, int*, int4*, int*, int*, u64*, uint8_t*, int, int*, long long*, int, u64*);
 
 InitFn init_fn(bool heavy, int sub)
 {
@@ -900,7 +921,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
         const int blocks = (int)std::min<long long>(((long long)size + tile - 1) / tile, round_grid);
         hipLaunchKernelGGL(sel_first, dim3(blocks), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[cur ^ 1],
                            &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot,
-                           ctx->d_cand);
+                           ctx->d_cand, (const u64*)nullptr);
       }
       while (size > 0) {
         hipLaunchKernelGGL(k_join_plain, dim3((size + 255) / 256), dim3(256), 0, s, wl[cur], size, ctx->d_parent,
@@ -909,7 +930,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
         const int blocks = (int)std::min<long long>(((long long)size + tile - 1) / tile, round_grid);
         hipLaunchKernelGGL(sel, dim3(blocks), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[cur ^ 1],
                            &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot,
-                           (u64*)nullptr);
+                           (u64*)nullptr, (const u64*)nullptr);
         cur ^= 1;
         HIPCHK(hipMemcpyAsync(&size, &d_cnt[cur], 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -928,13 +949,16 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       ctx->h_flag[slot] = -1;
       if (r == 0)
         hipLaunchKernelGGL(sel_first, dim3(round_grid), dim3(waves * kWave), 0, s, wl[0], &d_cnt[0], wl[1], &d_cnt[1],
-                           ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot, ctx->d_cand);
+                           ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot, ctx->d_cand,
+                           (const u64*)nullptr);
       // the first join reads the candidate bitmasks, later ones the record bits
       hipLaunchKernelGGL(join, dim3(round_grid), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt], &d_cnt[nxt],
                          ctx->d_parent, ctx->d_minv, d_inMST, tail_max, ctx->d_flag + slot, d_stats, rounds_slot,
-                         (const u64*)(r == 0 ? ctx->d_cand : nullptr));
+                         r == 0 ? ctx->d_cand : (u64*)nullptr);
+      // the select after the first join drops the records that join joined
       hipLaunchKernelGGL(sel, dim3(round_grid), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt], &d_cnt[nxt],
-                         ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot, (u64*)nullptr);
+                         ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot, (u64*)nullptr,
+                         (const u64*)(r == 0 ? ctx->d_cand : nullptr));
       HIPCHK(hipGetLastError());
       slot++;
       if (fixed) {
