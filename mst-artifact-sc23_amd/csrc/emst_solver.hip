@@ -586,7 +586,7 @@ template <int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void k_join(
     int4* __restrict__ wl, int* __restrict__ cnt, int4* __restrict__ wl_other, int* __restrict__ cnt_other,
     int* __restrict__ parent, u64* __restrict__ minv, uint8_t* __restrict__ inMST, int tail_max,
-    int* __restrict__ hflag, long long* __restrict__ stats, int rounds_slot, u64* __restrict__ cand)
+    int* __restrict__ hflag, long long* __restrict__ stats, int rounds_slot, const u64* cand, u64* joined_mask)
 {
   const int n = *cnt;
   if (n > 0 && n <= tail_max) {
@@ -632,14 +632,13 @@ __global__ __launch_bounds__(WAVES * kWave) void k_join(
         joined = true;
       }
     }
-    if (cand) {
-      // a joined record lies inside one tree from now on: its bit in the
-      // first mask (read above by this wavefront only) now tells the next
-      // select to drop it without finds; lane 0 holds the row's first
-      // record, so it is active whenever any lane is
-      const u64 jm = __ballot(joined);
-      if (lane_id() == 0) cand[i / kWave] = jm;
-    }
+    // A joined record lies inside one tree from now on: its bit tells the
+    // next select to drop it without finds.  In the first round the mask
+    // overwrites the candidate word this wavefront alone has just read.
+    // Lane 0 holds the row's first record, so it is active whenever any
+    // lane is.
+    const u64 jm = __ballot(joined);
+    if (lane_id() == 0) joined_mask[i / kWave] = jm;
   }
 }
 
@@ -684,7 +683,7 @@ typedef void (*InitFn)(const int*, const int*, const int*, const int*, int, int,
                        const int*);
 typedef void (*SelFn)(const int4*, const int*, int4*, int*, const int*, u64*, int, int, long long*, int, u64*,
                       const u64*);
-typedef void (*JoinFn)(int4*, int*, int4*, int*, int*, u64*, uint8_t*, int, int*, long long*, int, u64*);
+typedef void (*JoinFn)(int4*, int*, int4*, int*, int*, u64*, uint8_t*, int, int*, long long*, int, const u64*, u64*);
 
 InitFn init_fn(bool heavy, int sub)
 {
@@ -951,14 +950,14 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
         hipLaunchKernelGGL(sel_first, dim3(round_grid), dim3(waves * kWave), 0, s, wl[0], &d_cnt[0], wl[1], &d_cnt[1],
                            ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot, ctx->d_cand,
                            (const u64*)nullptr);
-      // the first join reads the candidate bitmasks, later ones the record bits
+      // the first join reads the candidate bitmasks; every join marks what it joined
       hipLaunchKernelGGL(join, dim3(round_grid), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt], &d_cnt[nxt],
                          ctx->d_parent, ctx->d_minv, d_inMST, tail_max, ctx->d_flag + slot, d_stats, rounds_slot,
-                         r == 0 ? ctx->d_cand : (u64*)nullptr);
-      // the select after the first join drops the records that join joined
+                         (const u64*)(r == 0 ? ctx->d_cand : nullptr), ctx->d_cand);
+      // each select drops the records the join before it joined
       hipLaunchKernelGGL(sel, dim3(round_grid), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt], &d_cnt[nxt],
                          ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot, (u64*)nullptr,
-                         (const u64*)(r == 0 ? ctx->d_cand : nullptr));
+                         (const u64*)ctx->d_cand);
       HIPCHK(hipGetLastError());
       slot++;
       if (fixed) {
