@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_zero(uint8_t* __restrict__ p, long long
   if (t < head) p[t] = 0;
   if (t < bytes - tail0) p[tail0 + t] = 0;
   int4* q = reinterpret_cast<int4*>(p + head);
-  for (long long i = t; i < n16; i += stride) q[i] = make_int4(0, 0, 0, 0);
+  for (long long i = t; i < n16; i += stride) st_stream(&q[i], make_int4(0, 0, 0, 0));
 }
 
 // ---------------------------------------------------------------------------
@@ -628,7 +628,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_join(
       if (wy) minv[y] = kNoKey;
       if (wx || wy) {
         hook(x, y, parent);
-        inMST[r.w] = 1;
+        __builtin_nontemporal_store((uint8_t)1, &inMST[r.w]);  // a scattered single-use byte
         joined = true;
       }
     }
