@@ -7,15 +7,18 @@ solves (needs a HIP device).
    candidate quantile q (threshold = sorted sample of rank floor(q*1023);
    the last candidate is the unfiltered path).  Each candidate's cost is
    modelled from the solve's own exact round schedule:
-       cost = LAUNCH_US * rounds + bytes / BW
-   where bytes is the schedule's byte count (initialisation passes,
-   records, 24 B per record entering a round, 96 B per survivor).  The
-   label is log2 of the cheapest q.  Two cost models give two label sets:
-     single  LAUNCH_US per round: one graph per solve, so every round's
-             fixed cost is the graph's own;
+       cost = launch_us * rounds + init_bytes / bw_init + round_bytes / bw_round
+   where init_bytes counts the setup and initialisation passes and their
+   records, and round_bytes 24 B per record entering a round and 96 B per
+   survivor.  The label is log2 of the cheapest q.  Two cost models give
+   two label sets:
+     single  LAUNCH_US per round, every byte at BW: one graph per solve,
+             so every round's fixed cost is the graph's own;
      union   LAUNCH_US / UNION_SHARE per round: in a batched arena about
              UNION_SHARE graphs share every round, so a graph's marginal
-             cost is its bytes.
+             cost is its bytes; round bytes are taken at UNION_ROUND_BW
+             (slower rates, 0.3-2 TB/s, left the labels almost unchanged
+             and did not make the batch faster).
 3. Full-batch Adam on the device (emst_mlp_train), one network per cost
    model, then an evaluation on held-out graphs against the plain rule
    (ECL_MST.cu:432-443) under the same cost model.
@@ -39,6 +42,22 @@ QS = [2.0 ** (-k / 2.0) for k in range(14, 0, -1)] + [1.0]   # 2^-7 .. 2^-0.5, t
 LAUNCH_US = 8.0
 UNION_SHARE = 1000.0
 BW = 3.0e12
+UNION_ROUND_BW = 3.0e12
+
+
+class CostModel:
+    def __init__(self, launch_us, bw_init, bw_round):
+        self.launch_us, self.bw_init, self.bw_round = launch_us, bw_init, bw_round
+
+    def cost(self, rounds, init_bytes, round_bytes):
+        return self.launch_us * rounds + (init_bytes / self.bw_init + round_bytes / self.bw_round) * 1e6
+
+    def as_dict(self):
+        return {"launch_us": self.launch_us, "bw_init": self.bw_init, "bw_round": self.bw_round}
+
+
+MODELS = {"single": CostModel(LAUNCH_US, BW, BW),
+          "union": CostModel(LAUNCH_US / UNION_SHARE, BW, UNION_ROUND_BW)}
 
 
 def make_graph(i, seed):
@@ -64,21 +83,17 @@ def make_graph(i, seed):
 
 
 def schedule_bytes(g, st, m_up_heavy):
+    """(initialisation bytes, round bytes) of a solve's schedule."""
     n, m = g.nodes, g.edges
     P = 2 if st.filter_used else 1
-    b = 12 * n + m + P * (4 * (n + 1) + 6 * m) + 16 * st.light
+    b = 12 * n + m + P * (4 * (n + 1) + 6 * m) + 16 * st.light + 5 * st.mst_edges
     if st.filter_used:
         b += 4 * n + 4 * m_up_heavy + 16 * st.heavy
-    b += 24 * st.round_in_sum + 96 * st.round_out_sum + 5 * st.mst_edges
-    return b
+    return b, 24 * st.round_in_sum + 96 * st.round_out_sum
 
 
-def cost_from(rounds, nbytes, launch_us):
-    return launch_us * rounds + nbytes / BW * 1e6
-
-
-def cost_of(g, st, m_up_heavy, launch_us=LAUNCH_US):
-    return cost_from(st.light_rounds + st.heavy_rounds, schedule_bytes(g, st, m_up_heavy), launch_us)
+def cost_of(g, st, m_up_heavy, model):
+    return model.cost(st.light_rounds + st.heavy_rounds, *schedule_bytes(g, st, m_up_heavy))
 
 
 def label_graph(solver, g):
@@ -94,35 +109,35 @@ def label_graph(solver, g):
             T = int(samples[int(math.floor(q * 1023))])
             st = solver.solve(emst.FILTER_PLAIN, T)
             heavy_up = int(len(upw) - np.searchsorted(upw, T, side="right"))
-        rows.append((st.light_rounds + st.heavy_rounds, schedule_bytes(g, st, heavy_up)))
+        rows.append((st.light_rounds + st.heavy_rounds, *schedule_bytes(g, st, heavy_up)))
     return feat, rows
 
 
-def best_q(rows, launch_us):
-    costs = [cost_from(r, b, launch_us) for r, b in rows]
+def best_q(rows, model):
+    costs = [model.cost(r, bi, br) for r, bi, br in rows]
     return QS[int(np.argmin(costs))]
 
 
-def plain_cost(solver, g, launch_us):
+def plain_cost(solver, g, model):
     uf, T = emst.plain_threshold(g)
     st = solver.solve(emst.FILTER_PLAIN if uf else emst.FILTER_OFF, T)
     upw = np.sort(g.eweight[g.upper_mask()])
     hu = int(len(upw) - np.searchsorted(upw, T, side="right")) if uf else 0
-    return cost_of(g, st, hu, launch_us), st
+    return cost_of(g, st, hu, model), st
 
 
-def evaluate(solver, params, launch_us, holdout, seed):
+def evaluate(solver, params, model, holdout, seed):
     """Modelled cost of the learned and the plain thresholds on held-out graphs."""
     solver.set_filter_model(params)
     ev = []
     for i in range(holdout):
         kind, g = make_graph(10_000_000 + i, seed + 1)
         solver.load(g)
-        pc, pst = plain_cost(solver, g, launch_us)
+        pc, pst = plain_cost(solver, g, model)
         lst = solver.solve(emst.FILTER_LEARNED, 0)
         upw = np.sort(g.eweight[g.upper_mask()])
         hu = int(len(upw) - np.searchsorted(upw, lst.threshold_used, side="right")) if lst.filter_used else 0
-        lc = cost_of(g, lst, hu, launch_us)
+        lc = cost_of(g, lst, hu, model)
         ev.append({"kind": kind, "n": g.nodes, "m": g.edges, "plain_cost": pc, "learned_cost": lc,
                    "plain_rounds": pst.light_rounds + pst.heavy_rounds,
                    "learned_rounds": lst.light_rounds + lst.heavy_rounds, "plain_light": pst.light,
@@ -146,30 +161,32 @@ def main():
     t0 = time.time()
     if a.costs:
         z = np.load(a.costs)
-        X, R, B = z["X"], z["rounds"], z["bytes"]
+        X, R, BI, BR = z["X"], z["rounds"], z["init_bytes"], z["round_bytes"]
         kinds = [str(k) for k in z["kinds"]]
     else:
-        X, R, B, kinds = [], [], [], []
+        X, R, BI, BR, kinds = [], [], [], [], []
         for i in range(a.graphs):
             kind, g = make_graph(i, a.seed)
             feat, rows = label_graph(solver, g)
             X.append(feat)
-            R.append([r for r, _ in rows])
-            B.append([b for _, b in rows])
+            R.append([r for r, _, _ in rows])
+            BI.append([bi for _, bi, _ in rows])
+            BR.append([br for _, _, br in rows])
             kinds.append(kind)
             if i % 50 == 0:
                 print(f"[train_filter] labelled {i}/{a.graphs} ({time.time()-t0:.0f}s) last {kind} n={g.nodes}",
                       flush=True)
         X = np.array(X, np.float32)
         R = np.array(R, np.int64)
-        B = np.array(B, np.float64)
-        np.savez(os.path.join(a.out, "filter_costs.npz"), X=X, rounds=R, bytes=B, kinds=np.array(kinds),
-                 qs=np.array(QS))
+        BI = np.array(BI, np.float64)
+        BR = np.array(BR, np.float64)
+        np.savez(os.path.join(a.out, "filter_costs.npz"), X=X, rounds=R, init_bytes=BI, round_bytes=BR,
+                 kinds=np.array(kinds), qs=np.array(QS))
     info = {"params": emst.MLP_PARAMS, "graphs": int(len(X)), "seed": a.seed, "epochs": a.epochs, "lr": a.lr,
-            "candidates": QS, "bw": BW, "models": {}}
-    for name, launch_us, fname in (("single", LAUNCH_US, "filter_mlp.f32"),
-                                   ("union", LAUNCH_US / UNION_SHARE, "filter_mlp_union.f32")):
-        qs = [best_q(list(zip(R[i], B[i])), launch_us) for i in range(len(X))]
+            "candidates": QS, "models": {}}
+    for name, fname in (("single", "filter_mlp.f32"), ("union", "filter_mlp_union.f32")):
+        model = MODELS[name]
+        qs = [best_q(list(zip(R[i], BI[i], BR[i])), model) for i in range(len(X))]
         Y = np.log2(np.array(qs, np.float32))
         p0 = emst.init_params(a.seed)
         t1 = time.time()
@@ -177,9 +194,9 @@ def main():
         print(f"[train_filter] {name}: trained {a.epochs} epochs in {time.time()-t1:.1f}s, final mse {loss:.4f} "
               f"(label variance {float(np.var(Y)):.4f})", flush=True)
         params.astype(np.float32).tofile(os.path.join(a.out, fname))
-        ratio, ev = evaluate(solver, params, launch_us, a.holdout, a.seed)
+        ratio, ev = evaluate(solver, params, model, a.holdout, a.seed)
         print(f"[train_filter] {name}: held-out geomean learned/plain modelled cost = {ratio:.3f}", flush=True)
-        info["models"][name] = {"file": fname, "launch_us": launch_us, "final_mse": loss,
+        info["models"][name] = {"file": fname, **model.as_dict(), "final_mse": loss,
                                 "label_variance": float(np.var(Y)), "holdout_geomean_cost_ratio": ratio,
                                 "holdout": ev, "label_q_histogram": {str(q): int(sum(1 for x in qs if x == q))
                                                                      for q in QS}}

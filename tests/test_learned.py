@@ -127,19 +127,19 @@ def test_trained_model_beats_plain_rule(which):
         pytest.skip("no trained model committed")
     sys.path.insert(0, os.path.join(ROOT, "mst-artifact-sc23_amd"))
     import train_filter as TF
-    launch_us = TF.LAUNCH_US if which == "single" else TF.LAUNCH_US / TF.UNION_SHARE
+    model = TF.MODELS[which]
     s = emst.Solver(0)
     s.set_filter_model(np.fromfile(path, np.float32))
     ratios = []
     for i in range(24):
         kind, g = TF.make_graph(20_000_000 + i, 99)
         s.load(g)
-        pc, pst = TF.plain_cost(s, g, launch_us)
+        pc, pst = TF.plain_cost(s, g, model)
         got, lst = s.gpu_mst(g, emst.FILTER_LEARNED, 0)
         assert np.array_equal(got, O.kruskal(g))
         upw = np.sort(g.eweight[g.upper_mask()])
         hu = int(len(upw) - np.searchsorted(upw, lst.threshold_used, side="right")) if lst.filter_used else 0
-        lc = TF.cost_of(g, lst, hu, launch_us)
+        lc = TF.cost_of(g, lst, hu, model)
         print(f"{kind} n={g.nodes} plain {pc:.1f} learned {lc:.1f} q={lst.learned_q:.4f} "
               f"light {pst.light}->{lst.light} rounds {pst.light_rounds + pst.heavy_rounds}->"
               f"{lst.light_rounds + lst.heavy_rounds}", flush=True)
