@@ -245,6 +245,18 @@ uint16_t emst_bf16_bits(float x);
 int emst_read_egr(const char* path, emst_graph_buf* out);          /* ECLgraph.h:37-64 */
 int emst_write_egr(const char* path, const emst_graph* g);         /* ECLgraph.h:66-80 */
 void emst_free_graph(emst_graph_buf* g);                           /* ECLgraph.h:82-90 */
+/* Formats of the reference's external baselines, written from a weighted
+ * CSR (the layouts of its converters, <baseline>/inputs/ecl2..._genweights.cpp):
+ *   emst_write_mtx     MatrixMarket real symmetric, lower triangle, 1-based
+ *                      (cuGraph_GPU/inputs/ecl2mtx_genweights_cugraph.cpp)
+ *   emst_write_pbbs    PBBS WeightedEdgeArray text (PBBS_CPU/inputs/ecl2pbbs_genweights.cpp)
+ *   emst_write_galois  Galois/Lonestar binary, version 1 (Lonestar_CPU/inputs/ecl2lsg_genweights.cpp)
+ *   emst_write_uminho  BoruvkaUMinho edge list (UMinho_BOTH/inputs/ecl2UMinho_genweights.cpp)
+ * EMST_E_ARG without weights, EMST_E_OPEN / EMST_E_WRITE on file errors. */
+int emst_write_mtx(const char* path, const emst_graph* g);
+int emst_write_pbbs(const char* path, const emst_graph* g);
+int emst_write_galois(const char* path, const emst_graph* g);
+int emst_write_uminho(const char* path, const emst_graph* g);
 /* Weights for weightless inputs, ECL_MST.cu:410-419: eweight[j] =
  * |1 + ((i*k) % nodes)| with a wrapping 32-bit multiply.  Allocates
  * g->eweight (freed by emst_free_graph). */

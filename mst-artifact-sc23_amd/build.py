@@ -1,8 +1,10 @@
-"""Builds libemst.so (gfx950 HIP kernels + C++ host code) and the ecl_mst CLI.
+"""Builds libemst.so (gfx950 HIP kernels + C++ host code) and the CLIs.
 
 Outputs, in-tree so they travel with the source tree:
   mst-artifact-sc23_amd/lib/libemst.so
-  mst-artifact-sc23_amd/bin/ecl_mst
+  mst-artifact-sc23_amd/bin/ecl_mst      reference CLI protocol (ECL_MST.cu:398-455)
+  mst-artifact-sc23_amd/bin/ecl_gen      seeded synthetic inputs as .egr files
+  mst-artifact-sc23_amd/bin/ecl_convert  .egr to the external baselines' formats
 Object files go to mst-artifact-sc23_amd/_build/.
 Usage: python mst-artifact-sc23_amd/build.py [-j N]
 """
@@ -27,9 +29,9 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Wno-unused-command-line-argument"]
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-Wno-unused-function"]
 
-HOST_SOURCES = ["egr_io.cpp", "generators.cpp", "cpu_mst.cpp"]
+HOST_SOURCES = ["egr_io.cpp", "generators.cpp", "cpu_mst.cpp", "formats.cpp"]
 HIP_SOURCES = ["emst_solver.hip", "learned.hip", "batch.hip"]
-CLI_SOURCES = ["ecl_mst_cli.cpp"]
+CLI_SOURCES = ["ecl_mst_cli.cpp", "ecl_gen_cli.cpp", "ecl_convert_cli.cpp"]
 
 
 def _run(cmd):
@@ -61,8 +63,8 @@ def build(jobs=8):
     tmp = lib + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-pthread"])
     os.replace(tmp, lib)
-    cli = os.path.join(BIN, "ecl_mst")
-    _run([CXX, "-o", cli] + cli_objs + [f"-L{LIB}", "-lemst", "-Wl,-rpath,$ORIGIN/../lib", "-pthread"])
+    for name, obj in zip(("ecl_mst", "ecl_gen", "ecl_convert"), cli_objs):
+        _run([CXX, "-o", os.path.join(BIN, name), obj, f"-L{LIB}", "-lemst", "-Wl,-rpath,$ORIGIN/../lib", "-pthread"])
     return lib
 
 

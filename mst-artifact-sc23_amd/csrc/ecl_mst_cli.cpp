@@ -7,14 +7,37 @@
 // Reader failures print the reference's messages (ECLgraph.h:42-59) and
 // exit with -1, as the reference does.  A JSON line with the solver's
 // statistics goes to stderr, leaving stdout in the reference's format.
+// EMST_FILTER=learned in the environment replaces the reference's threshold
+// rule by the trained network (filter_mlp.f32 next to the bin/ folder).
+
+#include <unistd.h>
 
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "../../include/emst.h"
+
+// The trained threshold network shipped beside the executable's bin/ folder.
+static bool load_network(std::vector<float>& p)
+{
+  char exe[4096];
+  const ssize_t n = readlink("/proc/self/exe", exe, sizeof(exe) - 1);
+  if (n <= 0) return false;
+  exe[n] = 0;
+  std::string dir(exe);
+  dir = dir.substr(0, dir.rfind('/'));   // .../bin
+  dir = dir.substr(0, dir.rfind('/'));   // package folder
+  FILE* f = std::fopen((dir + "/filter_mlp.f32").c_str(), "rb");
+  if (!f) return false;
+  p.resize(EMST_MLP_PARAMS);
+  const bool ok = std::fread(p.data(), 4, p.size(), f) == p.size();
+  std::fclose(f);
+  return ok;
+}
 
 static const char* reader_message(int rc)
 {
@@ -61,12 +84,21 @@ int main(int argc, char* argv[])
     fprintf(stderr, "ERROR %d: %s\n", rc, emst_last_error(ctx));
     exit(-1);
   }
+  const char* mode = std::getenv("EMST_FILTER");
+  const bool learned = mode && std::strcmp(mode, "learned") == 0;
+  if (learned) {
+    std::vector<float> p;
+    if (!load_network(p) || emst_set_filter_model(ctx, p.data()) != EMST_OK) {
+      fprintf(stderr, "ERROR: could not load filter_mlp.f32\n\n");
+      exit(-1);
+    }
+  }
   const int runs = 9;  // ECL_MST.cu:49
   emst_stats st{};
   for (int i = 0; i < runs; i++) {
-    int use_filter = 0;
+    int use_filter = EMST_FILTER_LEARNED;
     int32_t threshold = INT_MAX;
-    emst_plain_threshold(&g, &use_filter, &threshold);
+    if (!learned) emst_plain_threshold(&g, &use_filter, &threshold);
     if ((rc = emst_solve(ctx, use_filter, threshold, &st)) != EMST_OK) {
       fprintf(stderr, "ERROR %d: %s\n", rc, emst_last_error(ctx));
       exit(-1);

@@ -105,6 +105,8 @@ _solve_device = _sig("emst_solve_device", C.c_int, C.c_void_p, C.c_int32, C.c_in
                      C.c_void_p, C.c_int, C.c_int32, C.c_void_p, C.c_void_p, C.POINTER(Stats))
 _read_egr = _sig("emst_read_egr", C.c_int, C.c_char_p, C.POINTER(_GraphBuf))
 _write_egr = _sig("emst_write_egr", C.c_int, C.c_char_p, C.POINTER(_Graph))
+_writers = {fmt: _sig(f"emst_write_{fmt}", C.c_int, C.c_char_p, C.POINTER(_Graph))
+            for fmt in ("mtx", "pbbs", "galois", "uminho")}
 _free_graph = _sig("emst_free_graph", None, C.POINTER(_GraphBuf))
 _synth_weights = _sig("emst_synth_weights", C.c_int, C.POINTER(_GraphBuf))
 _mst_weight = _sig("emst_mst_weight", C.c_uint64, C.POINTER(_Graph), _pu8)
@@ -211,6 +213,11 @@ def read_egr(path, synth_weights=False):
 
 def write_egr(path, g):
     _check(_write_egr(os.fsencode(path), C.byref(g.view())), path)
+
+
+def write_format(path, g, fmt):
+    """One of the reference's baseline formats: "mtx", "pbbs", "galois" or "uminho"."""
+    _check(_writers[fmt](os.fsencode(path), C.byref(g.view())), path)
 
 
 def synth_weights(g):

@@ -238,3 +238,101 @@ def test_schedule_fixture_grid512_recomputed():
     # closed forms: a connected grid has n - 1 forest edges and one component
     assert ent["runs"][0]["mst_edges"] == g.nodes - 1 and ent["runs"][0]["components"] == 1
     assert O.schedule(g, 0, emst.INT_MAX)[1].light == g.edges // 2
+
+
+def _fmt_graph():
+    # 5 vertices, a multi-edge, an isolated vertex; stored one direction
+    # only, so the directed edge count is odd (the Galois pad word)
+    ni, nl, ew = csr_from_edges(5, [(0, 1, 7), (1, 2, 3), (2, 0, 9), (2, 3, 3), (0, 1, 2)], symmetric=False)
+    return emst.Graph(ni, nl, ew)
+
+
+def test_format_writers(tmp_path):
+    """The baseline formats, parsed back field by field (layouts of the
+    reference's ecl2*_genweights.cpp converters)."""
+    g = _fmt_graph()
+    assert g.edges % 2 == 1
+    src = np.repeat(np.arange(g.nodes), np.diff(g.nindex))
+    # MatrixMarket: lower triangle, 1-based, one decimal
+    p = tmp_path / "g.mtx"
+    emst.write_format(str(p), g, "mtx")
+    lines = p.read_text().splitlines()
+    assert lines[0] == "%%MatrixMarket matrix coordinate real symmetric"
+    assert lines[1] == f"{g.nodes} {g.nodes} {g.edges // 2}"
+    want = [f"{s + 1} {d + 1} {float(w):.1f}" for s, d, w in zip(src, g.nlist, g.eweight) if s > d]
+    assert lines[2:] == want
+    # PBBS weighted edge array: every directed edge, %f weights
+    p = tmp_path / "g.pbbs"
+    emst.write_format(str(p), g, "pbbs")
+    lines = p.read_text().splitlines()
+    assert lines[0] == "WeightedEdgeArray"
+    assert lines[1:] == [f"{s} {d} {float(w):f}" for s, d, w in zip(src, g.nlist, g.eweight)]
+    # UMinho edge list
+    p = tmp_path / "g.um"
+    emst.write_format(str(p), g, "uminho")
+    lines = p.read_text().splitlines()
+    assert lines[0] == f"{g.nodes} {g.edges}"
+    assert lines[1:] == [f"{s} {d} {w}" for s, d, w in zip(src, g.nlist, g.eweight)]
+    # Galois/Lonestar binary v1
+    p = tmp_path / "g.gr"
+    emst.write_format(str(p), g, "galois")
+    b = p.read_bytes()
+    head = np.frombuffer(b[:32], np.uint64)
+    assert list(head) == [1, 4, g.nodes, g.edges]
+    off = 32
+    ends = np.frombuffer(b[off:off + 8 * g.nodes], np.uint64)
+    assert np.array_equal(ends, g.nindex[1:])
+    off += 8 * g.nodes
+    assert np.array_equal(np.frombuffer(b[off:off + 4 * g.edges], np.int32), g.nlist)
+    off += 4 * g.edges + (4 if g.edges % 2 else 0)
+    assert np.array_equal(np.frombuffer(b[off:], np.int32), g.eweight)
+    # no weights: refused
+    with pytest.raises(emst.EmstError):
+        emst.write_format(str(tmp_path / "x.mtx"), emst.Graph(g.nindex, g.nlist, None), "mtx")
+
+
+def test_gen_and_convert_tools(tmp_path):
+    """bin/ecl_gen writes the generators' graphs; bin/ecl_convert adds the
+    reference's synthetic weights to a weightless input and converts."""
+    bindir = os.path.join(os.path.dirname(os.path.dirname(__file__)), "mst-artifact-sc23_amd", "bin")
+    out = tmp_path / "grid.egr"
+    r = subprocess.run([os.path.join(bindir, "ecl_gen"), "grid", "16", "24", "1", "1", str(1 << 20), str(out)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    g = emst.read_egr(str(out))
+    ref = emst.gen_grid(16, 24, 1, 1, 1 << 20)
+    assert np.array_equal(g.nindex, ref.nindex) and np.array_equal(g.nlist, ref.nlist)
+    assert np.array_equal(g.eweight, ref.eweight)
+    bare = tmp_path / "bare.egr"
+    emst.write_egr(str(bare), emst.Graph(ref.nindex, ref.nlist, None))
+    conv = tmp_path / "bare.um"
+    r = subprocess.run([os.path.join(bindir, "ecl_convert"), str(bare), "uminho", str(conv)], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr
+    assert "added missing weights using ECL-MST method" in r.stdout
+    w = np.array([int(x.split()[2]) for x in conv.read_text().splitlines()[1:]], np.int32)
+    assert np.array_equal(w, emst.synth_weights(emst.Graph(ref.nindex, ref.nlist, None)).eweight)
+    r = subprocess.run([os.path.join(bindir, "ecl_convert"), str(bare), "nosuch", str(conv)], capture_output=True,
+                       text=True)
+    assert r.returncode != 0
+
+
+def test_tables_from_capture(tmp_path):
+    """The table generator's arithmetic (median of 9, edges / median / 1e6)
+    on a capture in the CLI's protocol."""
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(__file__)), "mst-artifact-sc23_amd"))
+    import tables
+    g = emst.gen_grid(8, 8, 1, 1, 100)
+    path = tmp_path / "grid.egr"
+    emst.write_egr(str(path), g)
+    times = [0.5, 0.1, 0.9, 0.3, 0.2, 0.4, 0.6, 0.8, 0.7]
+    cap = f"input: {path}\n" + "".join(f"{t:12.9f}\n" for t in times)
+    rows = tables.parse_capture(cap)
+    assert rows == [(str(path), times)]
+    runtime, thru = tables.write_tables(str(tmp_path), [("ecl_mst", rows)], tables.egr_edges)
+    assert runtime["ecl_mst"][str(path)] == 0.5
+    assert abs(thru["ecl_mst"][str(path)] - g.edges / 0.5 / 1e6) < 1e-12
+    text = (tmp_path / "comparison_throughputs.csv").read_text().splitlines()
+    assert text[0] == ",ecl_mst, " and text[1].startswith("grid,")
+    with pytest.raises(ValueError):
+        tables.parse_capture(f"input: {path}\n 0.1\n")
