@@ -3,6 +3,9 @@ solves (needs a HIP device).
 
 1. Generates N seeded graphs: 2D grids, random geometric graphs and RMAT,
    2^12..2^20 vertices (log-uniform), mixed degrees and weight ranges.
+   The union network also sees Erdos-Renyi graphs (--union-er of them):
+   the batched workload mixes them in, and a network that never saw one
+   chose q near 0.5 where about 0.2 is best.
 2. For each graph: device features and weight samples, then one solve per
    candidate quantile q (threshold = sorted sample of rank floor(q*1023);
    the last candidate is the unfiltered path).  Each candidate's cost is
@@ -60,9 +63,11 @@ MODELS = {"single": CostModel(LAUNCH_US, BW, BW),
           "union": CostModel(LAUNCH_US / UNION_SHARE, BW, UNION_ROUND_BW)}
 
 
-def make_graph(i, seed):
+def make_graph(i, seed, kind=None):
+    """Graph i of the seeded mix: kind cycles grid, geometric, RMAT unless
+    given (3 selects Erdos-Renyi)."""
     rng = np.random.default_rng([seed, i])
-    kind = i % 3
+    kind = i % 3 if kind is None else kind
     logn = rng.uniform(12, 20)
     n = int(2 ** logn)
     wbits = int(rng.choice([6, 10, 16, 20, 24]))
@@ -75,6 +80,9 @@ def make_graph(i, seed):
     if kind == 1:
         deg = float(rng.choice([3.0, 4.0, 6.0, 8.0, 12.0]))
         return "rgg", emst.gen_rgg(n, deg, gseed, 1 << max(wbits, 8))
+    if kind == 3:
+        deg = float(rng.choice([4.0, 6.0, 10.0, 16.0]))
+        return "er", emst.gen_er(n, deg, gseed, 1, 1 << wbits)
     scale = int(round(logn))
     ef = int(rng.choice([4, 8, 16]))
     a = float(rng.uniform(0.45, 0.6))
@@ -126,12 +134,14 @@ def plain_cost(solver, g, model):
     return cost_of(g, st, hu, model), st
 
 
-def evaluate(solver, params, model, holdout, seed):
-    """Modelled cost of the learned and the plain thresholds on held-out graphs."""
+def evaluate(solver, params, model, holdout, seed, er_every=0):
+    """Modelled cost of the learned and the plain thresholds on held-out graphs
+    (every er_every-th one Erdos-Renyi when er_every > 0)."""
     solver.set_filter_model(params)
     ev = []
     for i in range(holdout):
-        kind, g = make_graph(10_000_000 + i, seed + 1)
+        er = er_every > 0 and i % er_every == er_every - 1
+        kind, g = make_graph(10_000_000 + i, seed + 1, 3 if er else None)
         solver.load(g)
         pc, pst = plain_cost(solver, g, model)
         lst = solver.solve(emst.FILTER_LEARNED, 0)
@@ -155,48 +165,56 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", default="gpurun_out/filter")
     ap.add_argument("--costs", default=None, help="reuse a filter_costs.npz instead of solving the graphs again")
+    ap.add_argument("--union-er", type=int, default=667, help="Erdos-Renyi graphs added for the union network")
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
     solver = emst.Solver(0)
     t0 = time.time()
+    X, R, BI, BR, kinds = [], [], [], [], []
     if a.costs:
+        # Rows already labelled (the first graphs of the same seeded mix);
+        # only the missing ones are solved.
         z = np.load(a.costs)
-        X, R, BI, BR = z["X"], z["rounds"], z["init_bytes"], z["round_bytes"]
+        X, R, BI, BR = (list(z[k]) for k in ("X", "rounds", "init_bytes", "round_bytes"))
         kinds = [str(k) for k in z["kinds"]]
-    else:
-        X, R, BI, BR, kinds = [], [], [], [], []
-        for i in range(a.graphs):
-            kind, g = make_graph(i, a.seed)
-            feat, rows = label_graph(solver, g)
-            X.append(feat)
-            R.append([r for r, _, _ in rows])
-            BI.append([bi for _, bi, _ in rows])
-            BR.append([br for _, _, br in rows])
-            kinds.append(kind)
-            if i % 50 == 0:
-                print(f"[train_filter] labelled {i}/{a.graphs} ({time.time()-t0:.0f}s) last {kind} n={g.nodes}",
-                      flush=True)
-        X = np.array(X, np.float32)
-        R = np.array(R, np.int64)
-        BI = np.array(BI, np.float64)
-        BR = np.array(BR, np.float64)
-        np.savez(os.path.join(a.out, "filter_costs.npz"), X=X, rounds=R, init_bytes=BI, round_bytes=BR,
-                 kinds=np.array(kinds), qs=np.array(QS))
+    total = a.graphs + a.union_er
+    for i in range(len(X), total):
+        kind, g = make_graph(i, a.seed, 3 if i >= a.graphs else None)
+        feat, rows = label_graph(solver, g)
+        X.append(feat)
+        R.append([r for r, _, _ in rows])
+        BI.append([bi for _, bi, _ in rows])
+        BR.append([br for _, _, br in rows])
+        kinds.append(kind)
+        if i % 50 == 0:
+            print(f"[train_filter] labelled {i}/{total} ({time.time()-t0:.0f}s) last {kind} n={g.nodes}", flush=True)
+    X = np.array(X[:total], np.float32)
+    R = np.array(R[:total], np.int64)
+    BI = np.array(BI[:total], np.float64)
+    BR = np.array(BR[:total], np.float64)
+    kinds = kinds[:total]
+    np.savez(os.path.join(a.out, "filter_costs.npz"), X=X, rounds=R, init_bytes=BI, round_bytes=BR,
+             kinds=np.array(kinds), qs=np.array(QS))
     info = {"params": emst.MLP_PARAMS, "graphs": int(len(X)), "seed": a.seed, "epochs": a.epochs, "lr": a.lr,
             "candidates": QS, "models": {}}
-    for name, fname in (("single", "filter_mlp.f32"), ("union", "filter_mlp_union.f32")):
+    # The single network trains on the first --graphs rows (grid, geometric,
+    # RMAT); the union network on every row, Erdos-Renyi included.
+    for name, fname, rows_used, er_every in (("single", "filter_mlp.f32", a.graphs, 0),
+                                             ("union", "filter_mlp_union.f32", total, 4 if a.union_er else 0)):
         model = MODELS[name]
-        qs = [best_q(list(zip(R[i], BI[i], BR[i])), model) for i in range(len(X))]
+        qs = [best_q(list(zip(R[i], BI[i], BR[i])), model) for i in range(rows_used)]
         Y = np.log2(np.array(qs, np.float32))
         p0 = emst.init_params(a.seed)
         t1 = time.time()
-        params, loss = emst.mlp_train(p0, X, Y, a.epochs, a.lr, a.seed)
+        params, loss = emst.mlp_train(p0, X[:rows_used], Y, a.epochs, a.lr, a.seed)
         print(f"[train_filter] {name}: trained {a.epochs} epochs in {time.time()-t1:.1f}s, final mse {loss:.4f} "
               f"(label variance {float(np.var(Y)):.4f})", flush=True)
         params.astype(np.float32).tofile(os.path.join(a.out, fname))
-        ratio, ev = evaluate(solver, params, model, a.holdout, a.seed)
+        ratio, ev = evaluate(solver, params, model, a.holdout, a.seed, er_every)
         print(f"[train_filter] {name}: held-out geomean learned/plain modelled cost = {ratio:.3f}", flush=True)
-        info["models"][name] = {"file": fname, **model.as_dict(), "final_mse": loss,
+        info["models"][name] = {"file": fname, "graphs": rows_used,
+                                "train_kinds": {k: kinds[:rows_used].count(k) for k in sorted(set(kinds[:rows_used]))},
+                                **model.as_dict(), "final_mse": loss,
                                 "label_variance": float(np.var(Y)), "holdout_geomean_cost_ratio": ratio,
                                 "holdout": ev, "label_q_histogram": {str(q): int(sum(1 for x in qs if x == q))
                                                                      for q in QS}}
