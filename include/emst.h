@@ -107,6 +107,30 @@ typedef struct {
 #define EMST_F_NO_GUARD 2   /* unconditional atomicMin (ablation of ECL_MST.cu:222-223) */
 #define EMST_F_NO_COMBINE 4 /* no per-workgroup LDS min-combining of keys (ablation)     */
 
+/* The de-optimization ladder (reference: Deoptimizations/ECL_MST_No*.cu,
+ * removing one optimization per file).  EMST_F_LADDER runs the reference's
+ * kernel sequence instead of the fused schedule: initWL, then kernel1,
+ * kernel2 and kernel3 per round with the worklist size read back by the host
+ * after every kernel1 (ECL_MST.cu:304-340).  Each EMST_F_NO_* bit below
+ * removes one optimization from that sequence and implies EMST_F_LADDER;
+ * EMST_F_NO_GUARD and the filter mode apply to both schedules.  Any
+ * combination gives the same inMST bytes as the default schedule.  Filter
+ * mode EMST_FILTER_LEARNED and the batched entry points take none of these
+ * bits. */
+#define EMST_F_LADDER 8              /* reference kernel sequence (ECL_MST.cu:304-340)             */
+#define EMST_F_NO_WARP 16            /* one thread walks every adjacency in initWL; no wave        */
+                                     /* cooperation at degree >= 4 (ECL_MST.cu:167, 180-205)       */
+#define EMST_F_NO_IMPLICIT_PC 32     /* kernel1 keeps the original endpoints; kernel2 and kernel3  */
+                                     /* find roots again with path splitting (ECL_MST.cu:218-219)  */
+#define EMST_F_NO_ONE_DIRECTION 64   /* both directions of every edge enter the worklist           */
+                                     /* (ECL_MST.cu:170, 195)                                      */
+#define EMST_F_NO_TUPLES 128         /* worklist of edge ids plus a source array instead of        */
+                                     /* {x, y, weight, j} records (ECL_MST.cu:176, 201)            */
+#define EMST_F_NO_DATA_DRIVEN 256    /* every round revisits the phase's whole edge set; a flag    */
+                                     /* ends the loop (ECL_MST.cu:313-323)                         */
+#define EMST_F_NO_EDGE_BASED 512     /* every round loops over each vertex's adjacency; no         */
+                                     /* worklist at all (ECL_MST.cu:210-251)                       */
+
 /* Filter modes for the use_filter argument of the solve entry points. */
 #define EMST_FILTER_OFF 0     /* gpuMST<false>(g, INT_MAX)                                  */
 #define EMST_FILTER_PLAIN 1   /* gpuMST<true>(g, threshold), threshold from the caller       */

@@ -9,9 +9,15 @@
 // statistics goes to stderr, leaving stdout in the reference's format.
 // EMST_FILTER=learned in the environment replaces the reference's threshold
 // rule by the trained network (filter_mlp.f32 next to the bin/ folder).
+// EMST_VARIANT=<suffixes> runs a rung of the de-optimization ladder, named
+// by the suffixes of the reference's variant binaries
+// (Deoptimizations/run_all_ECL-MST_deopts.sh: ecl_mst_NAG_NW_NF_...):
+// REF (the reference's kernel sequence, no removal), NAG, NW, NF, NIPC, NOD,
+// NT, NDD, NEB, joined by '_'.
 
 #include <unistd.h>
 
+#include <algorithm>
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
@@ -37,6 +43,41 @@ static bool load_network(std::vector<float>& p)
   const bool ok = std::fread(p.data(), 4, p.size(), f) == p.size();
   std::fclose(f);
   return ok;
+}
+
+// Flags and filter switch of a ladder variant name; false on an unknown suffix.
+static bool parse_variant(const char* name, int* flags, bool* no_filter)
+{
+  static const struct {
+    const char* tok;
+    int flag;
+  } table[] = {{"REF", 0},
+               {"NAG", EMST_F_NO_GUARD},
+               {"NW", EMST_F_NO_WARP},
+               {"NF", 0},
+               {"NIPC", EMST_F_NO_IMPLICIT_PC},
+               {"NOD", EMST_F_NO_ONE_DIRECTION},
+               {"NT", EMST_F_NO_TUPLES},
+               {"NDD", EMST_F_NO_DATA_DRIVEN},
+               {"NEB", EMST_F_NO_EDGE_BASED}};
+  *flags = EMST_F_LADDER;
+  *no_filter = false;
+  std::string s(name);
+  size_t pos = 0;
+  while (pos <= s.size()) {
+    const size_t end = std::min(s.find('_', pos), s.size());
+    const std::string tok = s.substr(pos, end - pos);
+    bool found = false;
+    for (const auto& t : table)
+      if (tok == t.tok) {
+        *flags |= t.flag;
+        if (tok == "NF") *no_filter = true;
+        found = true;
+      }
+    if (!found) return false;
+    pos = end + 1;
+  }
+  return true;
 }
 
 static const char* reader_message(int rc)
@@ -93,12 +134,28 @@ int main(int argc, char* argv[])
       exit(-1);
     }
   }
+  const char* variant = std::getenv("EMST_VARIANT");
+  bool no_filter = false;
+  if (variant && *variant) {
+    emst_config cfg{};
+    if (learned || !parse_variant(variant, &cfg.flags, &no_filter)) {
+      fprintf(stderr, "ERROR: bad EMST_VARIANT %s (learned filter not allowed)\n\n", variant);
+      exit(-1);
+    }
+    if ((rc = emst_set_config(ctx, &cfg)) != EMST_OK) {
+      fprintf(stderr, "ERROR %d: %s\n", rc, emst_last_error(ctx));
+      exit(-1);
+    }
+  }
   const int runs = 9;  // ECL_MST.cu:49
   emst_stats st{};
   for (int i = 0; i < runs; i++) {
     int use_filter = EMST_FILTER_LEARNED;
     int32_t threshold = INT_MAX;
-    if (!learned) emst_plain_threshold(&g, &use_filter, &threshold);
+    if (no_filter)
+      use_filter = EMST_FILTER_OFF;  // gpuMST<false>(g, INT_MAX), as the NoFilter variants call it
+    else if (!learned)
+      emst_plain_threshold(&g, &use_filter, &threshold);
     if ((rc = emst_solve(ctx, use_filter, threshold, &st)) != EMST_OK) {
       fprintf(stderr, "ERROR %d: %s\n", rc, emst_last_error(ctx));
       exit(-1);

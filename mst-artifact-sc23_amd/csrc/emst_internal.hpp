@@ -43,6 +43,13 @@ struct emst_ctx {
   long long* d_report = nullptr;
   int* h_decide = nullptr;            // host-mapped learned filter decision
   int* d_decide = nullptr;
+  // de-optimization ladder (ladder.hip): worklists, per-edge sources, counters
+  void* d_lad[2] = {nullptr, nullptr};
+  size_t cap_lad[2] = {0, 0};
+  int* d_slist = nullptr;
+  size_t cap_slist = 0;
+  int* d_lad_cnt = nullptr;
+  size_t cap_lad_cnt = 0;
   hipEvent_t ev[8] = {};
   int32_t last_nodes = 0;
   // learned threshold
@@ -86,5 +93,17 @@ int ensure_buffers(emst_ctx* ctx, int32_t nodes, int32_t edges, long long wl_cap
 int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
               const int* d_eweight, int use_filter, int32_t threshold, uint8_t* d_inMST, hipStream_t s,
               long long wl_cap, const VertexThresholds* vt, emst_stats* stats, FixedSchedule* fixed = nullptr);
+
+// The reference's kernel sequence with switchable optimizations (the
+// EMST_F_LADDER and EMST_F_NO_* bits of the context's flags): one graph, a
+// given threshold or none.
+int run_ladder(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
+               const int* d_eweight, int use_filter, int32_t threshold, uint8_t* d_inMST, hipStream_t s,
+               long long wl_cap, emst_stats* stats);
+
+// Flags that select the ladder schedule.
+constexpr int kLadderFlags = EMST_F_LADDER | EMST_F_NO_WARP | EMST_F_NO_IMPLICIT_PC | EMST_F_NO_ONE_DIRECTION |
+                             EMST_F_NO_TUPLES | EMST_F_NO_DATA_DRIVEN | EMST_F_NO_EDGE_BASED;
+constexpr int kAllFlags = EMST_F_HOST_SYNC | EMST_F_NO_GUARD | EMST_F_NO_COMBINE | kLadderFlags;
 
 }  // namespace emst

@@ -828,6 +828,12 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
   if (nodes < 1 || edges < 0) return ctx_fail(ctx, EMST_E_ARG, "node or edge count too low");
   if (use_filter < 0 || use_filter > EMST_FILTER_LEARNED) return ctx_fail(ctx, EMST_E_ARG, "bad filter mode");
   const emst_config& cfg = ctx->cfg;
+  if (cfg.flags & kLadderFlags) {
+    if (use_filter == EMST_FILTER_LEARNED || vt || fixed)
+      return ctx_fail(ctx, EMST_E_ARG, "the ablation ladder takes one graph and a given threshold");
+    return run_ladder(ctx, nodes, edges, d_nindex, d_nlist, d_eweight, use_filter, threshold, d_inMST, s, wl_cap,
+                      stats);
+  }
   const int task_edges = cfg.tile_edges;
   const int sub = task_edges / kWave;
   const int waves = cfg.block / kWave;
@@ -1087,7 +1093,8 @@ void emst_destroy(emst_ctx* ctx)
   void* ptrs[] = {ctx->d_cand,   ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, ctx->d_inMST, ctx->d_parent,
                   ctx->d_minv,   ctx->d_wl[0], ctx->d_wl[1],   ctx->d_gsrc,  ctx->d_cnt,
                   ctx->d_stats,  ctx->d_vclass, ctx->d_mlp16,  ctx->d_thr,   ctx->d_q,
-                  ctx->d_feat,   ctx->d_samples};
+                  ctx->d_feat,   ctx->d_samples, ctx->d_lad[0], ctx->d_lad[1], ctx->d_slist,
+                  ctx->d_lad_cnt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
@@ -1116,6 +1123,7 @@ int emst_set_config(emst_ctx* ctx, const emst_config* cfg)
   if (cfg->tail_max) c.tail_max = cfg->tail_max;
   if (cfg->init_grid_per_cu) c.init_grid_per_cu = cfg->init_grid_per_cu;
   c.flags = cfg->flags;
+  if (c.flags & ~kAllFlags) return ctx_fail(ctx, EMST_E_ARG, "unknown flag bits");
   if (c.grid_per_cu < 1 || c.grid_per_cu > 64 || c.init_grid_per_cu < 1 || c.init_grid_per_cu > 64)
     return ctx_fail(ctx, EMST_E_ARG, "grid_per_cu out of range");
   if (c.stage_recs < 64 ||
@@ -1307,7 +1315,7 @@ int emst_autotune(emst_ctx* ctx, int use_filter, int32_t threshold, int32_t reps
     return EMST_OK;
   };
   emst_config cur = saved;
-  cur.flags &= ~EMST_F_HOST_SYNC;
+  cur.flags &= ~(EMST_F_HOST_SYNC | kLadderFlags);
   double cur_ms = 0;
   int rc = measure(cur, &cur_ms);
   if (rc) {

@@ -26,6 +26,29 @@ ERRORS = {
 F_HOST_SYNC = 1
 F_NO_GUARD = 2
 F_NO_COMBINE = 4
+# the de-optimization ladder (Deoptimizations/ECL_MST_No*.cu)
+F_LADDER = 8
+F_NO_WARP = 16
+F_NO_IMPLICIT_PC = 32
+F_NO_ONE_DIRECTION = 64
+F_NO_TUPLES = 128
+F_NO_DATA_DRIVEN = 256
+F_NO_EDGE_BASED = 512
+# Rungs in the order the reference removes them (generate_deoptimize_tables.py
+# names the variants by their suffix chain); the filter rung is the filter mode.
+LADDER = [
+    ("ECL-MST", F_LADDER, True),
+    ("NoAtomicGuards", F_LADDER | F_NO_GUARD, True),
+    ("NoWarp", F_LADDER | F_NO_GUARD | F_NO_WARP, True),
+    ("NoFilter", F_LADDER | F_NO_GUARD | F_NO_WARP, False),
+    ("NoImplicitPathCompression", F_LADDER | F_NO_GUARD | F_NO_WARP | F_NO_IMPLICIT_PC, False),
+    ("NoOneDirection", F_LADDER | F_NO_GUARD | F_NO_WARP | F_NO_IMPLICIT_PC | F_NO_ONE_DIRECTION, False),
+    ("NoTuples", F_LADDER | F_NO_GUARD | F_NO_WARP | F_NO_IMPLICIT_PC | F_NO_ONE_DIRECTION | F_NO_TUPLES, False),
+    ("NoDataDriven", F_LADDER | F_NO_GUARD | F_NO_WARP | F_NO_IMPLICIT_PC | F_NO_ONE_DIRECTION | F_NO_TUPLES
+     | F_NO_DATA_DRIVEN, False),
+    ("NoEdgeBased", F_LADDER | F_NO_GUARD | F_NO_WARP | F_NO_IMPLICIT_PC | F_NO_ONE_DIRECTION | F_NO_TUPLES
+     | F_NO_DATA_DRIVEN | F_NO_EDGE_BASED, False),
+]
 FILTER_OFF, FILTER_PLAIN, FILTER_LEARNED = 0, 1, 2
 MLP_PARAMS, MLP_FEATURES, WEIGHT_SAMPLES = 6337, 32, 1024
 INT_MAX = 2**31 - 1

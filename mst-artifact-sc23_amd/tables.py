@@ -13,8 +13,13 @@ the .egr file's header, not from a fixed list.
       the tables
   python tables.py table <out_dir> <capture.csv>...    tables from captures
       made earlier (each capture becomes one column)
+  python tables.py deopt <out_dir> <graph.egr>...    the de-optimization
+      ladder (run_all_ECL-MST_deopts.sh and generate_deoptimize_tables.py):
+      one capture per rung, ecl_mst_<suffixes>_out.csv, then
+      deopt_runtimes.csv and deopt_throughputs.csv
 
-Tables: <out_dir>/comparison_runtimes.csv and comparison_throughputs.csv.
+Tables: <out_dir>/comparison_runtimes.csv and comparison_throughputs.csv
+(deopt: deopt_runtimes.csv and deopt_throughputs.csv).
 """
 import os
 import statistics
@@ -25,6 +30,13 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CLI = os.path.join(HERE, "bin", "ecl_mst")
 RUNS = 9
+# Rungs of the ladder by the reference's binary suffixes
+# (Deoptimizations/run_all_ECL-MST_deopts.sh:7-15), preceded by the fused
+# schedule ("", the default CLI) and the reference's kernel sequence with no
+# removal (REF).  The reference's table leaves out NAG_NW
+# (generate_deoptimize_tables.py:6-13); this one keeps every rung.
+DEOPT_VARIANTS = ["", "REF", "NAG", "NAG_NW", "NAG_NW_NF", "NAG_NW_NF_NIPC", "NAG_NW_NF_NIPC_NOD",
+                  "NAG_NW_NF_NIPC_NOD_NT", "NAG_NW_NF_NIPC_NOD_NT_NDD", "NAG_NW_NF_NIPC_NOD_NT_NDD_NEB"]
 
 
 def egr_edges(path):
@@ -52,12 +64,12 @@ def parse_capture(text):
     return out
 
 
-def write_tables(out_dir, columns, edges_of):
+def write_tables(out_dir, columns, edges_of, prefix="comparison"):
     """columns: [(column name, [(input path, times)])]; one row per input of the first column."""
     inputs = [p for p, _ in columns[0][1]]
     runtime = {c: {p: statistics.median(t) for p, t in rows} for c, rows in columns}
     thru = {c: {p: edges_of(p) / runtime[c][p] / 1e6 for p in runtime[c]} for c, _ in columns}
-    for fname, table in (("comparison_runtimes.csv", runtime), ("comparison_throughputs.csv", thru)):
+    for fname, table in ((f"{prefix}_runtimes.csv", runtime), (f"{prefix}_throughputs.csv", thru)):
         with open(os.path.join(out_dir, fname), "w") as w:
             w.write("," + "".join(f"{c}, " for c, _ in columns))
             for p in inputs:
@@ -67,18 +79,36 @@ def write_tables(out_dir, columns, edges_of):
     return runtime, thru
 
 
+def capture(graphs, env):
+    """The CLI's stdout over every graph with extra environment env."""
+    text = ""
+    for g in graphs:
+        r = subprocess.run([CLI, g], stdout=subprocess.PIPE, text=True, env={**os.environ, **env}, timeout=3600)
+        if r.returncode != 0:
+            raise RuntimeError(f"{CLI} {g} failed with {r.returncode}")
+        if "ERROR" in r.stdout:
+            raise RuntimeError(f"{g}: the CLI reports a result mismatch")
+        text += r.stdout
+    return text
+
+
+def deopt(out_dir, graphs, variants=DEOPT_VARIANTS):
+    os.makedirs(out_dir, exist_ok=True)
+    columns = []
+    for v in variants:
+        col = "ecl_mst" + (f"_{v}" if v else "")
+        text = capture(graphs, {"EMST_VARIANT": v} if v else {})
+        with open(os.path.join(out_dir, f"{col}_out.csv"), "w") as f:
+            f.write(text)
+        columns.append((col, parse_capture(text)))
+    return write_tables(out_dir, columns, egr_edges, prefix="deopt")
+
+
 def run(out_dir, graphs):
     os.makedirs(out_dir, exist_ok=True)
     columns = []
     for col, env in (("ecl_mst", {}), ("ecl_mst_learned", {"EMST_FILTER": "learned"})):
-        text = ""
-        for g in graphs:
-            r = subprocess.run([CLI, g], stdout=subprocess.PIPE, text=True, env={**os.environ, **env}, timeout=3600)
-            if r.returncode != 0:
-                raise RuntimeError(f"{CLI} {g} failed with {r.returncode}")
-            if "ERROR" in r.stdout:
-                raise RuntimeError(f"{g}: the CLI reports a result mismatch")
-            text += r.stdout
+        text = capture(graphs, env)
         with open(os.path.join(out_dir, f"{col}_out.csv"), "w") as f:
             f.write(text)
         columns.append((col, parse_capture(text)))
@@ -86,12 +116,16 @@ def run(out_dir, graphs):
 
 
 def main(argv):
-    if len(argv) < 3 or argv[0] not in ("run", "table"):
+    if len(argv) < 3 or argv[0] not in ("run", "table", "deopt"):
         sys.stderr.write(__doc__)
         return 1
     out_dir = argv[1]
     if argv[0] == "run":
         run(out_dir, argv[2:])
+    elif argv[0] == "deopt":
+        deopt(out_dir, argv[2:])
+        print(f"wrote {out_dir}/deopt_runtimes.csv and {out_dir}/deopt_throughputs.csv")
+        return 0
     else:
         os.makedirs(out_dir, exist_ok=True)
         cols = []

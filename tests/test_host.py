@@ -45,6 +45,29 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(emst.lib, n)
 
 
+def test_flag_constants_match_header():
+    text = open(os.path.join(ROOT, "include", "emst.h")).read()
+    flags = {k: int(v) for k, v in re.findall(r"#define EMST_(F_\w+)\s+(\d+)", text)}
+    assert len(flags) == 10
+    for k, v in flags.items():
+        assert getattr(emst, k) == v, k
+    assert len(set(flags.values())) == len(flags) and all(v & (v - 1) == 0 for v in flags.values())
+
+
+def test_ladder_rungs_remove_one_optimization_each():
+    """The rung table follows the reference's variant chain
+    (Deoptimizations/ECL_MST_NoAtomicGuards_NoWarp_..._NoEdgeBased.cu):
+    each rung keeps the previous rung's removals and adds one."""
+    names = [n for n, _, _ in emst.LADDER]
+    assert names == ["ECL-MST", "NoAtomicGuards", "NoWarp", "NoFilter", "NoImplicitPathCompression",
+                     "NoOneDirection", "NoTuples", "NoDataDriven", "NoEdgeBased"]
+    for (_, f0, filt0), (name, f1, filt1) in zip(emst.LADDER, emst.LADDER[1:]):
+        assert f1 & f0 == f0, name
+        removed = bin(f1 ^ f0).count("1") + (filt0 != filt1)
+        assert removed == 1, name
+    assert emst.LADDER[0][1] == emst.F_LADDER and emst.LADDER[0][2]
+
+
 ORC_TO_EMST = {0: 0, -1: -10, -2: -11, -3: -12, -4: -13, -6: -14, -7: -15, -8: -16}
 
 

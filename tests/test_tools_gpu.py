@@ -33,3 +33,28 @@ def test_cli_modes_and_tables(tmp_path):
     lines = (tmp_path / "comparison_throughputs.csv").read_text().splitlines()
     assert lines[0] == ",ecl_mst, ecl_mst_learned, "
     assert [ln.split(",")[0] for ln in lines[1:]] == ["rmat12", "grid"]
+
+
+def test_cli_deopt_ladder(tmp_path):
+    """Every rung through the CLI (EMST_VARIANT) with the reference's
+    verification, and the deopt tables of generate_deoptimize_tables.py."""
+    import tables
+    egr = tmp_path / "rmat11.egr"
+    r = subprocess.run([os.path.join(PKG, "bin", "ecl_gen"), "rmat", "11", "8", "0.57", "0.19", "0.19", "3", "1",
+                        "1000", str(egr)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    print("step: tables.deopt", flush=True)
+    runtime, _ = tables.deopt(str(tmp_path), [str(egr)])
+    cols = ["ecl_mst" + (f"_{v}" if v else "") for v in tables.DEOPT_VARIANTS]
+    for col in cols:
+        cap = (tmp_path / f"{col}_out.csv").read_text()
+        assert "ERROR" not in cap and cap.count("input:") == 1, col
+        assert runtime[col][str(egr)] > 0
+    lines = (tmp_path / "deopt_throughputs.csv").read_text().splitlines()
+    assert lines[0] == "," + "".join(f"{c}, " for c in cols)
+    bad = subprocess.run([CLI_PATH, str(egr)], capture_output=True, text=True,
+                         env={**os.environ, "EMST_VARIANT": "NAG_XYZ"})
+    assert bad.returncode != 0 and "bad EMST_VARIANT" in bad.stderr
+
+
+CLI_PATH = os.path.join(PKG, "bin", "ecl_mst")
