@@ -322,6 +322,58 @@ def run_batch(a, emst):
     return out
 
 
+# Inputs of the ladder measurement: the shapes of the reference's inputs
+# that the generators reproduce (2d-2e20.sym exactly; rmat16.sym,
+# rmat22.sym and r4-2e23.sym in kind and size), with the reference's
+# synthetic weights (ECL_MST.cu:410-419).
+LADDER_GRAPHS = {
+    "2d-2e20": lambda emst: emst.gen_grid(1024, 1024, 1, 1, 2),
+    "rmat16": lambda emst: emst.gen_rmat(16, 8, 0.57, 0.19, 0.19, 2, 1, 2),
+    "rmat22": lambda emst: emst.gen_rmat(22, 8, 0.57, 0.19, 0.19, 2, 1, 2),
+    "r4-2e23": lambda emst: emst.gen_er(1 << 23, 8.0, 4, 1, 2),
+}
+
+
+def run_ladder(a, emst):
+    """Every rung of the de-optimization ladder (Deoptimizations/ECL_MST_No*.cu)
+    on the ladder inputs: median solve time of up to --steps solves per rung
+    (fewer once a rung has spent --ladder-budget seconds), each rung's inMST
+    compared with the fused schedule's."""
+    s = emst.Solver(0)
+    rows = {}
+    for name in a.ladder_graphs.split(","):
+        g = emst.synth_weights(LADDER_GRAPHS[name](emst))
+        log(f"ladder input {name}: n={g.nodes} m={g.edges}")
+        s.load(g)
+        uf, T = emst.plain_threshold(g)
+        rungs = [r for r in [("fused", 0, True)] + list(emst.LADDER)
+                 if a.ladder_rungs is None or r[0] in a.ladder_rungs.split(",")]
+        ref = None
+        row = {}
+        for rung, flags, filt in rungs:
+            s.set_config(flags=flags)
+            mode, thr = (emst.FILTER_PLAIN if uf else emst.FILTER_OFF, T) if filt else (emst.FILTER_OFF, emst.INT_MAX)
+            s.solve(mode, thr)
+            ts, t0 = [], time.perf_counter()
+            while len(ts) < a.steps and (time.perf_counter() - t0 < a.ladder_budget or len(ts) < 1):
+                ts.append(s.solve(mode, thr).ms)
+            got = s.fetch()
+            if ref is None:
+                ref = got
+            exact = bool(np.array_equal(got, ref))
+            ms = float(np.median(ts))
+            row[rung] = {"ms": round(ms, 4), "medges_s": round(g.edges / ms / 1e3, 1), "solves": len(ts),
+                         "exact": exact}
+            log(f"{name} {rung}: {ms:.3f} ms ({len(ts)} solves) exact={exact}")
+            if not exact:
+                raise SystemExit(f"{name} {rung}: inMST differs from the fused schedule")
+        s.set_config()
+        rows[name] = {"nodes": g.nodes, "edges": g.edges, "rungs": row}
+    s.close()
+    return {"metric": "de-optimization ladder: solve time per rung (device events, graph resident)",
+            "unit": "ms", "n_gpus": 1, "data": "synthetic", "weights": "ECL_MST.cu:410-419", "inputs": rows}
+
+
 def run_copy(a):
     """Calibration for HBM traffic counters: the HIP runtime's device-to-
     device copy of a known size (read 1 GiB, write 1 GiB per launch).  Run
@@ -353,7 +405,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=9)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="rmat24", choices=sorted(WORKLOADS) + ["batch", "copy"])
+    ap.add_argument("--workload", default="rmat24", choices=sorted(WORKLOADS) + ["batch", "copy", "ladder"])
     ap.add_argument("--scale", type=int, default=None, help="override the RMAT scale (testing only)")
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -364,6 +416,9 @@ def main():
     ap.add_argument("--batch-nmax", type=int, default=1000000)
     ap.add_argument("--loop-sample", type=int, default=256)
     ap.add_argument("--verify-graphs", type=int, default=32)
+    ap.add_argument("--ladder-graphs", default=",".join(LADDER_GRAPHS))
+    ap.add_argument("--ladder-budget", type=float, default=5.0, help="seconds of solves per rung before stopping")
+    ap.add_argument("--ladder-rungs", default=None, help="comma list of rung names (default: every rung)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "
@@ -373,7 +428,10 @@ def main():
         print(json.dumps(run_copy(a)))
         return
     import emst
-    out = run_batch(a, emst) if a.workload == "batch" else run_single(a, emst)
+    if a.workload == "ladder":
+        out = run_ladder(a, emst)
+    else:
+        out = run_batch(a, emst) if a.workload == "batch" else run_single(a, emst)
     print(json.dumps(out))
 
 
