@@ -368,6 +368,23 @@ def run_ladder(a, emst):
             if not exact:
                 raise SystemExit(f"{name} {rung}: inMST differs from the fused schedule")
         s.set_config()
+        if a.ladder_rungs is None or "contract" in a.ladder_rungs.split(","):
+            # the related-work baseline on the same input, checked by weight
+            if ref is None:
+                s.solve(emst.FILTER_PLAIN if uf else emst.FILTER_OFF, T)
+                ref = s.fetch()
+            s.contract_mst(g)
+            ts, t0 = [], time.perf_counter()
+            while len(ts) < a.steps and (time.perf_counter() - t0 < a.ladder_budget or len(ts) < 1):
+                got_c, st_c = s.contract_mst(g)
+                ts.append(st_c.ms)
+            same_w = int(st_c.weight) == int(g.eweight[ref.astype(bool)].astype(np.int64).sum())
+            ms = float(np.median(ts))
+            row["contract"] = {"ms": round(ms, 4), "medges_s": round(g.edges / ms / 1e3, 1), "solves": len(ts),
+                               "same_weight": same_w, "iterations": st_c.iterations}
+            log(f"{name} contract: {ms:.3f} ms ({len(ts)} solves) same_weight={same_w}")
+            if not same_w:
+                raise SystemExit(f"{name} contract: forest weight differs")
         rows[name] = {"nodes": g.nodes, "edges": g.edges, "rungs": row}
     s.close()
     return {"metric": "de-optimization ladder: solve time per rung (device events, graph resident)",

@@ -114,7 +114,9 @@ typedef struct {
  * after every kernel1 (ECL_MST.cu:304-340).  Each EMST_F_NO_* bit below
  * removes one optimization from that sequence and implies EMST_F_LADDER;
  * EMST_F_NO_GUARD and the filter mode apply to both schedules.  Any
- * combination gives the same inMST bytes as the default schedule.  Filter
+ * combination gives the same inMST bytes as the default schedule (with
+ * EMST_F_NO_ONE_DIRECTION, for a symmetric CSR, as the reference's inputs
+ * are).  Filter
  * mode EMST_FILTER_LEARNED and the batched entry points take none of these
  * bits. */
 #define EMST_F_LADDER 8              /* reference kernel sequence (ECL_MST.cu:304-340)             */
@@ -295,6 +297,25 @@ int emst_cpu_mst(const emst_graph* g, uint8_t* inMST);
  * the 64-bit weight sums agree, 1 when they differ, <0 on bad arguments. */
 int emst_verify(const emst_graph* g, const uint8_t* cpu, const uint8_t* gpu, uint64_t* cpu_weight,
                 uint64_t* gpu_weight);
+
+/* ---- Related-work GPU baseline ------------------------------------------
+ * Vertex-based Boruvka with graph contraction, the algorithm of the
+ * reference's UMinho GPU code (UMinho_BOTH/src/BoruvkaUMinho_GPU.cu:10-455),
+ * run on the context's device for comparison with gpuMST.  Loads g (as
+ * emst_load), times the contraction loop with device events, and marks a
+ * minimum spanning forest in inMST[edges]: its weight is the MSF weight, but
+ * under equal weights its edges can differ from gpuMST's, and either
+ * direction's position of an edge may be marked.  Weights compare as
+ * unsigned 32-bit values.  Needs a symmetric CSR, as the reference's inputs
+ * are: one-way edges can close a cycle of picks, reported as EMST_E_LIMIT. */
+typedef struct {
+  double ms;              /* device time of the contraction loop            */
+  int32_t iterations;     /* contraction iterations                         */
+  int32_t color_passes;   /* pointer-jumping passes, all iterations         */
+  int64_t mst_edges;      /* marked forest edges                            */
+  uint64_t weight;        /* their total weight, as unsigned 32-bit values  */
+} emst_contract_stats;
+int emst_contract_mst(emst_ctx* ctx, const emst_graph* g, uint8_t* inMST, emst_contract_stats* stats);
 
 /* Library version string. */
 const char* emst_version(void);

@@ -172,6 +172,12 @@ _batch_thresholds = _sig("emst_batch_thresholds", C.c_int, C.c_void_p, _p32, _p3
 _batch_set_model = _sig("emst_batch_set_filter_model", C.c_int, C.c_void_p, _pf32)
 _batch_info = _sig("emst_batch_info", C.c_int, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                    C.POINTER(C.c_int64))
+class ContractStats(C.Structure):
+    _fields_ = [("ms", C.c_double), ("iterations", C.c_int32), ("color_passes", C.c_int32),
+                ("mst_edges", C.c_int64), ("weight", C.c_uint64)]
+
+
+_contract = _sig("emst_contract_mst", C.c_int, C.c_void_p, C.POINTER(_Graph), _pu8, C.POINTER(ContractStats))
 _autotune = _sig("emst_autotune", C.c_int, C.c_void_p, C.c_int, C.c_int32, C.c_int32, C.POINTER(Config),
                  C.POINTER(C.c_double))
 
@@ -400,6 +406,15 @@ class Solver:
                                 C.c_void_p(d_inmst), C.c_void_p(stream) if stream else None, C.byref(st)))
         return st
 
+
+    def contract_mst(self, g):
+        """The related-work baseline (Boruvka with graph contraction, the
+        UMinho GPU algorithm): (inMST, ContractStats)."""
+        out = np.zeros(max(g.edges, 1), np.uint8)
+        st = ContractStats()
+        self._err(_contract(self._h, C.byref(g.view()), out.ctypes.data_as(_pu8), C.byref(st)))
+        self.graph = g
+        return out[:g.edges], st
 
     def autotune(self, use_filter, threshold, reps=3):
         """Runs the autotuner on the loaded graph; returns (config dict, ms)."""

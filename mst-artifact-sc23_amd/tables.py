@@ -9,8 +9,9 @@ the .egr file's header, not from a fixed list.
 
   python tables.py run <out_dir> <graph.egr>...    runs bin/ecl_mst on every
       graph, once with the reference's threshold rule (ecl_mst_out.csv) and
-      once with the trained network (ecl_mst_learned_out.csv), then writes
-      the tables
+      once with the trained network (ecl_mst_learned_out.csv), and
+      bin/contract_mst, the related-work baseline (UMinho_gpu_out.csv), then
+      writes the tables
   python tables.py table <out_dir> <capture.csv>...    tables from captures
       made earlier (each capture becomes one column)
   python tables.py deopt <out_dir> <graph.egr>...    the de-optimization
@@ -29,6 +30,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CLI = os.path.join(HERE, "bin", "ecl_mst")
+# the related-work baseline (Boruvka with graph contraction, UMinho GPU)
+CONTRACT_CLI = os.path.join(HERE, "bin", "contract_mst")
 RUNS = 9
 # Rungs of the ladder by the reference's binary suffixes
 # (Deoptimizations/run_all_ECL-MST_deopts.sh:7-15), preceded by the fused
@@ -79,13 +82,13 @@ def write_tables(out_dir, columns, edges_of, prefix="comparison"):
     return runtime, thru
 
 
-def capture(graphs, env):
+def capture(graphs, env, cli=CLI):
     """The CLI's stdout over every graph with extra environment env."""
     text = ""
     for g in graphs:
-        r = subprocess.run([CLI, g], stdout=subprocess.PIPE, text=True, env={**os.environ, **env}, timeout=3600)
+        r = subprocess.run([cli, g], stdout=subprocess.PIPE, text=True, env={**os.environ, **env}, timeout=3600)
         if r.returncode != 0:
-            raise RuntimeError(f"{CLI} {g} failed with {r.returncode}")
+            raise RuntimeError(f"{cli} {g} failed with {r.returncode}")
         if "ERROR" in r.stdout:
             raise RuntimeError(f"{g}: the CLI reports a result mismatch")
         text += r.stdout
@@ -107,8 +110,9 @@ def deopt(out_dir, graphs, variants=DEOPT_VARIANTS):
 def run(out_dir, graphs):
     os.makedirs(out_dir, exist_ok=True)
     columns = []
-    for col, env in (("ecl_mst", {}), ("ecl_mst_learned", {"EMST_FILTER": "learned"})):
-        text = capture(graphs, env)
+    for col, env, cli in (("ecl_mst", {}, CLI), ("ecl_mst_learned", {"EMST_FILTER": "learned"}, CLI),
+                          ("UMinho_gpu", {}, CONTRACT_CLI)):
+        text = capture(graphs, env, cli)
         with open(os.path.join(out_dir, f"{col}_out.csv"), "w") as f:
             f.write(text)
         columns.append((col, parse_capture(text)))
