@@ -116,6 +116,15 @@ int main(int argc, char* argv[])
     exit(-1);
   }
   const emst_graph g{gb.nodes, gb.edges, gb.nindex, gb.nlist, gb.eweight};
+  const char* mode = std::getenv("EMST_FILTER");
+  const bool learned = mode && std::strcmp(mode, "learned") == 0;
+  const char* variant = std::getenv("EMST_VARIANT");
+  bool no_filter = false;
+  emst_config vcfg{};
+  if (variant && *variant && (learned || !parse_variant(variant, &vcfg.flags, &no_filter))) {
+    fprintf(stderr, "ERROR: bad EMST_VARIANT %s (learned filter not allowed)\n\n", variant);
+    exit(-1);
+  }
   emst_ctx* ctx = nullptr;
   if (emst_create(0, &ctx) != EMST_OK) {
     fprintf(stderr, "ERROR: there is no HIP capable device\n\n");
@@ -125,8 +134,6 @@ int main(int argc, char* argv[])
     fprintf(stderr, "ERROR %d: %s\n", rc, emst_last_error(ctx));
     exit(-1);
   }
-  const char* mode = std::getenv("EMST_FILTER");
-  const bool learned = mode && std::strcmp(mode, "learned") == 0;
   if (learned) {
     std::vector<float> p;
     if (!load_network(p) || emst_set_filter_model(ctx, p.data()) != EMST_OK) {
@@ -134,15 +141,8 @@ int main(int argc, char* argv[])
       exit(-1);
     }
   }
-  const char* variant = std::getenv("EMST_VARIANT");
-  bool no_filter = false;
   if (variant && *variant) {
-    emst_config cfg{};
-    if (learned || !parse_variant(variant, &cfg.flags, &no_filter)) {
-      fprintf(stderr, "ERROR: bad EMST_VARIANT %s (learned filter not allowed)\n\n", variant);
-      exit(-1);
-    }
-    if ((rc = emst_set_config(ctx, &cfg)) != EMST_OK) {
+    if ((rc = emst_set_config(ctx, &vcfg)) != EMST_OK) {
       fprintf(stderr, "ERROR %d: %s\n", rc, emst_last_error(ctx));
       exit(-1);
     }

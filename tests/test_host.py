@@ -359,3 +359,36 @@ def test_tables_from_capture(tmp_path):
     assert text[0] == ",ecl_mst, " and text[1].startswith("grid,")
     with pytest.raises(ValueError):
         tables.parse_capture(f"input: {path}\n 0.1\n")
+
+
+def test_deopt_tables_from_captures(tmp_path):
+    """The deopt tables (generate_deoptimize_tables.py:64-84): one column per
+    rung capture, in the variant order, median of 9 and Medges/s."""
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(__file__)), "mst-artifact-sc23_amd"))
+    import tables
+    g = emst.gen_grid(8, 8, 1, 1, 100)
+    path = tmp_path / "grid.egr"
+    emst.write_egr(str(path), g)
+    cols = []
+    for k, v in enumerate(tables.DEOPT_VARIANTS):
+        times = [0.1 * (k + 1)] * 9
+        cols.append(("ecl_mst" + (f"_{v}" if v else ""), tables.parse_capture(
+            f"input: {path}\n" + "".join(f"{t:12.9f}\n" for t in times))))
+    runtime, thru = tables.write_tables(str(tmp_path), cols, tables.egr_edges, prefix="deopt")
+    head = (tmp_path / "deopt_runtimes.csv").read_text().splitlines()[0]
+    assert head == "," + "".join(f"{c}, " for c, _ in cols)
+    assert head.startswith(",ecl_mst, ecl_mst_REF, ecl_mst_NAG, ecl_mst_NAG_NW, ")
+    row = (tmp_path / "deopt_throughputs.csv").read_text().splitlines()[1].split(",")
+    assert row[0] == "grid" and abs(float(row[2]) - g.edges / 0.2 / 1e6) < 1e-9
+
+
+def test_cli_rejects_bad_variant_before_the_device(tmp_path):
+    g = emst.gen_grid(4, 4, 1, 1, 100)
+    path = tmp_path / "g.egr"
+    emst.write_egr(str(path), g)
+    for bad, extra in (("NAG_XYZ", {}), ("", {}), ("NAG_NW", {"EMST_FILTER": "learned"})):
+        env = {**os.environ, "EMST_VARIANT": bad, **extra}
+        r = subprocess.run([CLI, str(path)], capture_output=True, text=True, env=env)
+        if bad == "":
+            continue  # an empty name selects the default schedule (needs the device)
+        assert r.returncode == 255 and "bad EMST_VARIANT" in r.stderr, (bad, r.stderr)
