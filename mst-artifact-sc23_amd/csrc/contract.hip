@@ -514,7 +514,7 @@ extern "C" int emst_contract_mst(emst_ctx* ctx, const emst_graph* g, uint8_t* in
       CCHK(hipMemcpy(hb.data(), b_big.p, (size_t)nb * sizeof(int4), hipMemcpyDeviceToHost));
       for (int i = 0; i < nb; i++)
         for (int c = hb[i].y; c < hb[i].z; c += std::min(kChunk, hb[i].z - c))
-          hc.push_back(make_int4(i, c, std::min(c + kChunk, hb[i].z), hb[i].x));
+          hc.push_back(make_int4(i, c, (int)std::min<long long>((long long)c + kChunk, hb[i].z), hb[i].x));
       if (hc.size() > chmax) return ctx_fail(ctx, EMST_E_LIMIT, "chunk table overflow");
       nchunks = (int)hc.size();
       CCHK(hipMemcpy(b_chunks.p, hc.data(), hc.size() * sizeof(int4), hipMemcpyHostToDevice));
