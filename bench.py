@@ -182,14 +182,22 @@ def run_single(a, emst):
     # once at its natural size) and the compulsory bytes (read the CSR once,
     # write inMST once), both against the median device time.
     mst_edges = int(np.count_nonzero(learned_inmst))
+    # the reference schedule keeps parallel records: its worklist sizes come
+    # from one more (untimed) solve with that elimination switched off
+    cfg0 = s.config().as_dict()
+    s.set_config(**{**cfg0, "flags": cfg0["flags"] | emst.F_NO_PAIR_DEDUP})
+    st_ref = s.solve(emst.FILTER_LEARNED, 0)
+    s.set_config(**cfg0)
     P = 2 if stl.filter_used else 1
     model = 12 * n + m + P * (4 * (n + 1) + 6 * m) + 16 * stl.light
     if stl.filter_used:
         model += 4 * n + 4 * heavy_up + 16 * stl.heavy
-    model += 24 * stl.round_in_sum + 96 * stl.round_out_sum + 5 * mst_edges
+    model += 24 * st_ref.round_in_sum + 96 * st_ref.round_out_sum + 5 * mst_edges
     compulsory = 9 * m + 4 * (n + 1)
     t_med = statistics.median(x.ms for x in st_l) * 1e-3
-    solve_roof = {"model_bytes": model, "model_GBps": model / t_med / 1e9, "model_frac": model / t_med / 1e9 / PEAK_HBM_GBS,
+    solve_roof = {"round_records": {"reference_in_out": [st_ref.round_in_sum, st_ref.round_out_sum],
+                                    "with_pair_elimination_in_out": [stl.round_in_sum, stl.round_out_sum]},
+                  "model_bytes": model, "model_GBps": model / t_med / 1e9, "model_frac": model / t_med / 1e9 / PEAK_HBM_GBS,
                   "compulsory_bytes": compulsory, "compulsory_GBps": compulsory / t_med / 1e9,
                   "compulsory_frac": compulsory / t_med / 1e9 / PEAK_HBM_GBS, "median_device_ms": t_med * 1e3}
     out = {

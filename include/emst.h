@@ -106,6 +106,13 @@ typedef struct {
 #define EMST_F_HOST_SYNC 1  /* read the worklist size back every round, as ECL_MST.cu:318 */
 #define EMST_F_NO_GUARD 2   /* unconditional atomicMin (ablation of ECL_MST.cu:222-223) */
 #define EMST_F_NO_COMBINE 4 /* no per-workgroup LDS min-combining of keys (ablation)     */
+/* Keep parallel records.  By default, after a phase's first round, each
+ * k_select tile keeps only the lightest of the records that join the same
+ * two components (the others close a cycle with it and cannot be in the
+ * forest).  Results and round counts are unchanged; with this bit the
+ * worklist sizes (round_in_sum, round_out_sum) are the reference schedule's
+ * (ECL_MST.cu:210-226 keeps every crossing record). */
+#define EMST_F_NO_PAIR_DEDUP 1024
 
 /* The de-optimization ladder (reference: Deoptimizations/ECL_MST_No*.cu,
  * removing one optimization per file).  EMST_F_LADDER runs the reference's

@@ -104,6 +104,6 @@ int run_ladder(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex,
 // Flags that select the ladder schedule.
 constexpr int kLadderFlags = EMST_F_LADDER | EMST_F_NO_WARP | EMST_F_NO_IMPLICIT_PC | EMST_F_NO_ONE_DIRECTION |
                              EMST_F_NO_TUPLES | EMST_F_NO_DATA_DRIVEN | EMST_F_NO_EDGE_BASED;
-constexpr int kAllFlags = EMST_F_HOST_SYNC | EMST_F_NO_GUARD | EMST_F_NO_COMBINE | kLadderFlags;
+constexpr int kAllFlags = EMST_F_HOST_SYNC | EMST_F_NO_GUARD | EMST_F_NO_COMBINE | EMST_F_NO_PAIR_DEDUP | kLadderFlags;
 
 }  // namespace emst

@@ -51,6 +51,7 @@ constexpr int kMarkSpan = 256;   // longest vertex range of a task resolved by m
 constexpr int kHflagSlots = 512;  // host-visible round results per solve
 constexpr int kSelItems = 4;      // records per thread per k_select tile
 constexpr int kTbl = 1024;        // LDS min-combining table entries
+constexpr int kPair = 1024;       // LDS parallel-record table entries (k_select after the first round)
 constexpr int kReportWords = ST_COUNT + 3;  // counters, threshold, filter flag, quantile bits
 
 // ---------------------------------------------------------------------------
@@ -409,11 +410,39 @@ __device__ __forceinline__ bool offer_key(int* tags, u64* keys, u64* minv, int r
 // crosses (both guaranteed by the initialisation), so finds and the copy
 // are skipped and the list is joined in place.
 // ---------------------------------------------------------------------------
+// Parallel records: after the first round, two records whose endpoints
+// have the same pair of roots connect the same two components, and of the
+// two only the lighter can be in the forest (it closes a cycle with the
+// heavier, which is that cycle's maximum).  Each tile therefore keeps, per
+// root pair, only its lightest record; records whose pair finds no LDS slot
+// within 8 probes are kept.  The joins of every round are unchanged (each
+// component's minimum record survives), so rounds and results are those of
+// the reference schedule; only the worklist sizes shrink.  With dedup = 0
+// every crossing record is kept and the sizes are the reference's.
+__device__ __forceinline__ int pair_slot(u64* tags, u64* keys, u64 pair, u64 key)
+{
+  const unsigned h = ((unsigned)(pair ^ (pair >> 29)) * 0x9E3779B1u) >> (32 - 10);
+#pragma unroll
+  for (int pr = 0; pr < 8; pr++) {
+    const unsigned sl = (h + pr) & (kPair - 1);
+    u64 t = tags[sl];
+    if (t == kNoKey) {
+      t = atomicCAS(&tags[sl], kNoKey, pair);
+      if (t == kNoKey) t = pair;
+    }
+    if (t == pair) {
+      atomicMin(&keys[sl], key);
+      return (int)sl;
+    }
+  }
+  return -1;
+}
+
 template <bool FIRST, int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void k_select(
     const int4* __restrict__ wl_in, const int* __restrict__ cnt_in, int4* __restrict__ wl_out,
     int* __restrict__ cnt_out, const int* __restrict__ parent, u64* __restrict__ minv, int guard, int combine,
-    long long* __restrict__ stats, int rounds_slot, u64* __restrict__ cand, const u64* __restrict__ dead)
+    int dedup, long long* __restrict__ stats, int rounds_slot, u64* __restrict__ cand, const u64* __restrict__ dead)
 {
   constexpr int BT = WAVES * kWave;
   constexpr int TILE = BT * kSelItems;
@@ -423,6 +452,8 @@ __global__ __launch_bounds__(WAVES * kWave) void k_select(
   __shared__ int s_wtot[WAVES];
   __shared__ int s_woff[WAVES];
   __shared__ int s_base;
+  __shared__ u64 s_ptag[FIRST ? 1 : kPair];
+  __shared__ u64 s_pkey[FIRST ? 1 : kPair];
   const int n_in = *cnt_in;
   if (n_in == 0) return;
   const int tid = threadIdx.x;
@@ -434,6 +465,11 @@ __global__ __launch_bounds__(WAVES * kWave) void k_select(
     s_tags[i] = -1;
     s_keys[i] = kNoKey;
   }
+  if (!FIRST && dedup)
+    for (int i = tid; i < kPair; i += BT) {
+      s_ptag[i] = kNoKey;
+      s_pkey[i] = kNoKey;
+    }
   __syncthreads();
   const int lane = lane_id();
   const int wid = tid / kWave;
@@ -481,6 +517,34 @@ __global__ __launch_bounds__(WAVES * kWave) void k_select(
       mine += keep[k];
     }
     if (FIRST) continue;
+    if (dedup) {
+      // keep the lightest record of each root pair in this tile
+      int slot[kSelItems];
+#pragma unroll
+      for (int k = 0; k < kSelItems; k++) {
+        slot[k] = -1;
+        if (keep[k]) {
+          const unsigned lo = (unsigned)min(r[k].x, r[k].y), hi = (unsigned)max(r[k].x, r[k].y);
+          slot[k] = pair_slot(s_ptag, s_pkey, ((u64)lo << 32) | hi, make_key(r[k].z, r[k].w));
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kSelItems; k++)
+        if (slot[k] >= 0 && s_pkey[slot[k]] != make_key(r[k].z, r[k].w)) {
+          keep[k] = false;
+          mine--;
+        }
+      __syncthreads();
+      // clear the slots this tile used (the scan below has a barrier
+      // before the next tile's inserts)
+#pragma unroll
+      for (int k = 0; k < kSelItems; k++)
+        if (slot[k] >= 0) {
+          s_ptag[slot[k]] = kNoKey;
+          s_pkey[slot[k]] = kNoKey;
+        }
+    }
     // workgroup exclusive scan of the per-thread survivor counts
     int x = mine;
 #pragma unroll
@@ -681,7 +745,7 @@ __global__ __launch_bounds__(256) void k_join_plain(const int4* __restrict__ wl,
 typedef void (*InitFn)(const int*, const int*, const int*, const int*, int, int, int, const int*,
                        const unsigned*, int4*, int*, int, long long*, const int*, const int*, const int*,
                        const int*);
-typedef void (*SelFn)(const int4*, const int*, int4*, int*, const int*, u64*, int, int, long long*, int, u64*,
+typedef void (*SelFn)(const int4*, const int*, int4*, int*, const int*, u64*, int, int, int, long long*, int, u64*,
                       const u64*);
 typedef void (*JoinFn)(int4*, int*, int4*, int*, int*, u64*, uint8_t*, int, int*, long long*, int, const u64*, u64*);
 
@@ -847,6 +911,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
     return ctx_fail(ctx, EMST_E_ARG, "host-sync mode takes one graph and a given threshold");
   const int guard = (cfg.flags & EMST_F_NO_GUARD) ? 0 : 1;
   const int combine = (cfg.flags & EMST_F_NO_COMBINE) ? 0 : 1;
+  const int dedup = (cfg.flags & EMST_F_NO_PAIR_DEDUP) ? 0 : 1;
   const int tail_max = cfg.tail_max < 0 ? 0 : cfg.tail_max;
   const int round_grid = std::max(1, ctx->cus * std::max(1, cfg.grid_per_cu));
   const int init_grid = std::max(1, ctx->cus * std::max(1, cfg.init_grid_per_cu));
@@ -925,7 +990,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       if (size > 0) {
         const int blocks = (int)std::min<long long>(((long long)size + tile - 1) / tile, round_grid);
         hipLaunchKernelGGL(sel_first, dim3(blocks), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[cur ^ 1],
-                           &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot,
+                           &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, combine, dedup, d_stats, rounds_slot,
                            ctx->d_cand, (const u64*)nullptr);
       }
       while (size > 0) {
@@ -934,7 +999,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
         HIPCHK(hipMemsetAsync(&d_cnt[cur ^ 1], 0, 4, s));
         const int blocks = (int)std::min<long long>(((long long)size + tile - 1) / tile, round_grid);
         hipLaunchKernelGGL(sel, dim3(blocks), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[cur ^ 1],
-                           &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot,
+                           &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, combine, dedup, d_stats, rounds_slot,
                            (u64*)nullptr, (const u64*)nullptr);
         cur ^= 1;
         HIPCHK(hipMemcpyAsync(&size, &d_cnt[cur], 4, hipMemcpyDeviceToHost, s));
@@ -954,7 +1019,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       ctx->h_flag[slot] = -1;
       if (r == 0)
         hipLaunchKernelGGL(sel_first, dim3(round_grid), dim3(waves * kWave), 0, s, wl[0], &d_cnt[0], wl[1], &d_cnt[1],
-                           ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot, ctx->d_cand,
+                           ctx->d_parent, ctx->d_minv, guard, combine, dedup, d_stats, rounds_slot, ctx->d_cand,
                            (const u64*)nullptr);
       // the first join reads the candidate bitmasks; every join marks what it joined
       hipLaunchKernelGGL(join, dim3(round_grid), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt], &d_cnt[nxt],
@@ -962,7 +1027,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
                          (const u64*)(r == 0 ? ctx->d_cand : nullptr), ctx->d_cand);
       // each select drops the records the join before it joined
       hipLaunchKernelGGL(sel, dim3(round_grid), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt], &d_cnt[nxt],
-                         ctx->d_parent, ctx->d_minv, guard, combine, d_stats, rounds_slot, (u64*)nullptr,
+                         ctx->d_parent, ctx->d_minv, guard, combine, dedup, d_stats, rounds_slot, (u64*)nullptr,
                          (const u64*)ctx->d_cand);
       HIPCHK(hipGetLastError());
       slot++;

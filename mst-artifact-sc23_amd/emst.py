@@ -26,6 +26,7 @@ ERRORS = {
 F_HOST_SYNC = 1
 F_NO_GUARD = 2
 F_NO_COMBINE = 4
+F_NO_PAIR_DEDUP = 1024
 # the de-optimization ladder (Deoptimizations/ECL_MST_No*.cu)
 F_LADDER = 8
 F_NO_WARP = 16

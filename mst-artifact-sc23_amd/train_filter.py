@@ -169,6 +169,10 @@ def main():
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
     solver = emst.Solver(0)
+    # Labels price the reference schedule's worklists: parallel-record
+    # elimination only trims them, and switching it off keeps the labels
+    # (and so the networks) independent of that optimisation.
+    solver.set_config(flags=emst.F_NO_PAIR_DEDUP)
     t0 = time.time()
     X, R, BI, BR, kinds = [], [], [], [], []
     if a.costs:

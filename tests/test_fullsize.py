@@ -49,19 +49,29 @@ def _modes(s, g):
 
 
 def check_schedules(s, g, cfg):
-    """The solve's deterministic counts against the oracle's schedule."""
+    """The solve's deterministic counts against the oracle's schedule: with
+    parallel records kept (the reference schedule) every count; with them
+    eliminated (the default) the same counts but worklist sizes no larger."""
     ent = SCHEDULES[cfg]
     assert (g.nodes, g.edges) == (ent["nodes"], ent["edges"]) and digest(g) == ent["sha256"]
     s.load(g)
+    saved = s.config().as_dict()
     for run in ent["runs"]:
-        st = s.solve(emst.FILTER_PLAIN if run["use_filter"] else emst.FILTER_OFF, run["threshold"])
-        got = (st.light, st.heavy, st.light_rounds, st.heavy_rounds, st.round_in_sum, st.round_out_sum,
-               emst.mst_weight(g, s.fetch()))
+        mode = emst.FILTER_PLAIN if run["use_filter"] else emst.FILTER_OFF
         want = (run["L"], run["H"], run["light_rounds"], run["heavy_rounds"],
                 sum(run["light_in"]) + sum(run["heavy_in"]), sum(run["light_out"]) + sum(run["heavy_out"]),
                 run["weight"])
+        s.set_config(**{**saved, "flags": saved["flags"] | emst.F_NO_PAIR_DEDUP})
+        st = s.solve(mode, run["threshold"])
+        got = (st.light, st.heavy, st.light_rounds, st.heavy_rounds, st.round_in_sum, st.round_out_sum,
+               emst.mst_weight(g, s.fetch()))
+        s.set_config(**saved)
         print(f"  schedule T={run['threshold']}: {got}", flush=True)
         assert got == want, (cfg, run["threshold"])
+        st = s.solve(mode, run["threshold"])
+        got = (st.light, st.heavy, st.light_rounds, st.heavy_rounds, emst.mst_weight(g, s.fetch()))
+        assert got == want[:4] + want[6:], (cfg, run["threshold"], "pair elimination")
+        assert st.round_in_sum <= want[4] and st.round_out_sum <= want[5]
 
 
 def test_c1_grid_schedule():

@@ -25,7 +25,21 @@ def G(ni, nl, ew):
     return emst.Graph(ni, nl, ew)
 
 
+def with_flags(solver, extra, fn):
+    """fn() under the solver's current configuration plus flag bits extra."""
+    saved = solver.config().as_dict()
+    solver.set_config(**{**saved, "flags": saved["flags"] | extra})
+    try:
+        return fn()
+    finally:
+        solver.set_config(**saved)
+
+
 def check(solver, g, use_filter=None, T=None, rounds=True, kruskal=True):
+    """The solve under the current configuration against the oracle: bytes,
+    L, H and round counts exactly; worklist sizes exactly with parallel
+    records kept (the reference schedule), and no larger with them
+    eliminated (the default)."""
     if use_filter is None:
         use_filter, T = emst.plain_threshold(g)
     print(f"case n={g.nodes} m={g.edges} filter={use_filter} T={T}", flush=True)
@@ -45,7 +59,12 @@ def check(solver, g, use_filter=None, T=None, rounds=True, kruskal=True):
         assert st.heavy_rounds == (sch.heavy_rounds if use_filter else 0)
         sin = sum(sch.light_w[i] for i in range(sch.light_rounds)) + sum(sch.heavy_w[i] for i in range(sch.heavy_rounds))
         sout = sum(sch.light_s[i] for i in range(sch.light_rounds)) + sum(sch.heavy_s[i] for i in range(sch.heavy_rounds))
-        assert (st.round_in_sum, st.round_out_sum) == (sin, sout)
+        got2, st2 = with_flags(solver, emst.F_NO_PAIR_DEDUP, lambda: solver.gpu_mst(g, use_filter, T))
+        assert np.array_equal(got2, got)
+        assert (st2.light_rounds, st2.heavy_rounds, st2.light, st2.heavy) == \
+            (st.light_rounds, st.heavy_rounds, st.light, st.heavy)
+        assert (st2.round_in_sum, st2.round_out_sum) == (sin, sout)
+        assert st.round_in_sum <= sin and st.round_out_sum <= sout
     assert st.mst_edges == sch.mst_edges
     return got, st
 
@@ -151,6 +170,7 @@ def test_modes_agree(solver):
     uf, T = emst.plain_threshold(g)
     base, _ = solver.gpu_mst(g, uf, T)
     for kw in (dict(flags=emst.F_HOST_SYNC), dict(flags=emst.F_NO_GUARD), dict(flags=emst.F_NO_COMBINE),
+               dict(flags=emst.F_NO_PAIR_DEDUP), dict(flags=emst.F_HOST_SYNC | emst.F_NO_PAIR_DEDUP),
                dict(flags=emst.F_NO_GUARD | emst.F_NO_COMBINE), dict(tail_max=-1),
                dict(tail_max=1 << 20), dict(stage_recs=64), dict(grid_per_cu=1)):
         print("mode", kw, flush=True)
@@ -243,3 +263,17 @@ def test_shapes_and_autotune_exact(solver):
     solver.solve(uf, T)
     assert np.array_equal(solver.fetch(), ref)
     solver.set_config()
+
+
+def test_pair_elimination_shrinks_worklists(solver):
+    """After the first round most records of a skewed graph join the same
+    few components; keeping only the lightest per root pair and tile leaves
+    the bytes and round counts alone and shrinks the worklists."""
+    g = emst.gen_rmat(16, 16, 0.57, 0.19, 0.19, 2, 1, 1 << 24)
+    uf, T = emst.plain_threshold(g)
+    got, st = solver.gpu_mst(g, uf, T)
+    ref, st_ref = with_flags(solver, emst.F_NO_PAIR_DEDUP, lambda: solver.gpu_mst(g, uf, T))
+    assert np.array_equal(got, ref) and np.array_equal(got, O.kruskal(g))
+    assert (st.light_rounds, st.heavy_rounds) == (st_ref.light_rounds, st_ref.heavy_rounds)
+    print("in/out sums", (st.round_in_sum, st.round_out_sum), "reference", (st_ref.round_in_sum, st_ref.round_out_sum))
+    assert st.round_in_sum < st_ref.round_in_sum and st.round_out_sum < st_ref.round_out_sum

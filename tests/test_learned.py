@@ -129,6 +129,7 @@ def test_trained_model_beats_plain_rule(which):
     import train_filter as TF
     model = TF.MODELS[which]
     s = emst.Solver(0)
+    s.set_config(flags=emst.F_NO_PAIR_DEDUP)  # the schedule the cost model was trained on
     s.set_filter_model(np.fromfile(path, np.float32))
     ratios = []
     for i in range(24):
