@@ -277,3 +277,31 @@ def test_pair_elimination_shrinks_worklists(solver):
     assert (st.light_rounds, st.heavy_rounds) == (st_ref.light_rounds, st_ref.heavy_rounds)
     print("in/out sums", (st.round_in_sum, st.round_out_sum), "reference", (st_ref.round_in_sum, st_ref.round_out_sum))
     assert st.round_in_sum < st_ref.round_in_sum and st.round_out_sum < st_ref.round_out_sum
+
+
+def test_parallel_edge_multigraphs(solver):
+    """Multigraphs dense in parallel edges and equal weights, where parallel
+    record elimination drops the most: clusters of vertices with many edges
+    between clusters, so later rounds carry many records per root pair.
+    Bytes, L, H and rounds against the oracle; worklist sizes with records
+    kept."""
+    rng = np.random.default_rng(17)
+    for it in range(6):
+        n = int(rng.integers(2000, 20000))
+        k = int(rng.integers(20, 200))                   # clusters of consecutive ids
+        m = 4 * n
+        u = rng.integers(0, n, m)
+        # 80% of the edges stay inside their cluster, the rest cross to a
+        # random vertex; each edge comes 1-3 times (parallel edges)
+        width = max(2, n // k)
+        inside = (u // width) * width + rng.integers(0, width, m)
+        v = np.where(rng.random(m) < 0.8, np.minimum(inside, n - 1), rng.integers(0, n, m))
+        rep = rng.integers(1, 4, m)
+        edges = []
+        for a, b, r in zip(u.tolist(), v.tolist(), rep.tolist()):
+            if a != b:
+                for _ in range(r):
+                    edges.append((a, b, int(rng.integers(1, 6))))
+        g = G(*csr_from_edges(n, edges))
+        for uf, T in ((True, 2), (True, 4), (False, emst.INT_MAX)):
+            check(solver, g, uf, T)
