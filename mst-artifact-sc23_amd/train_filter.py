@@ -2,7 +2,10 @@
 solves (needs a HIP device).
 
 1. Generates N seeded graphs: 2D grids, random geometric graphs and RMAT,
-   2^12..2^20 vertices (log-uniform), mixed degrees and weight ranges.
+   2^12..2^20 vertices (log-uniform), mixed degrees and weight ranges, and
+   --large more of 2^21..2^22.5 vertices for the single-solve network (it
+   is applied to graphs far above 2^20 vertices, and without them it
+   extrapolated the quantile badly on geometric and Erdos-Renyi graphs).
    The union network also sees Erdos-Renyi graphs (--union-er of them):
    the batched workload mixes them in, and a network that never saw one
    chose q near 0.5 where about 0.2 is best.
@@ -63,12 +66,13 @@ MODELS = {"single": CostModel(LAUNCH_US, BW, BW),
           "union": CostModel(LAUNCH_US / UNION_SHARE, BW, UNION_ROUND_BW)}
 
 
-def make_graph(i, seed, kind=None):
+def make_graph(i, seed, kind=None, logn_range=(12, 20)):
     """Graph i of the seeded mix: kind cycles grid, geometric, RMAT unless
-    given (3 selects Erdos-Renyi)."""
+    given (3 selects Erdos-Renyi); log2 of the vertex count is uniform in
+    logn_range."""
     rng = np.random.default_rng([seed, i])
     kind = i % 3 if kind is None else kind
-    logn = rng.uniform(12, 20)
+    logn = rng.uniform(*logn_range)
     n = int(2 ** logn)
     wbits = int(rng.choice([6, 10, 16, 20, 24]))
     gseed = int(rng.integers(1, 2**31))
@@ -166,6 +170,8 @@ def main():
     ap.add_argument("--out", default="gpurun_out/filter")
     ap.add_argument("--costs", default=None, help="reuse a filter_costs.npz instead of solving the graphs again")
     ap.add_argument("--union-er", type=int, default=667, help="Erdos-Renyi graphs added for the union network")
+    ap.add_argument("--large", type=int, default=48,
+                    help="graphs of 2^21-2^22.5 vertices added for the single-solve network")
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
     solver = emst.Solver(0)
@@ -181,9 +187,12 @@ def main():
         z = np.load(a.costs)
         X, R, BI, BR = (list(z[k]) for k in ("X", "rounds", "init_bytes", "round_bytes"))
         kinds = [str(k) for k in z["kinds"]]
-    total = a.graphs + a.union_er
+    total = a.graphs + a.union_er + a.large
     for i in range(len(X), total):
-        kind, g = make_graph(i, a.seed, 3 if i >= a.graphs else None)
+        if i >= a.graphs + a.union_er:
+            kind, g = make_graph(i, a.seed, None, (21, 22.5))
+        else:
+            kind, g = make_graph(i, a.seed, 3 if i >= a.graphs else None)
         feat, rows = label_graph(solver, g)
         X.append(feat)
         R.append([r for r, _, _ in rows])
@@ -201,23 +210,28 @@ def main():
              kinds=np.array(kinds), qs=np.array(QS))
     info = {"params": emst.MLP_PARAMS, "graphs": int(len(X)), "seed": a.seed, "epochs": a.epochs, "lr": a.lr,
             "candidates": QS, "models": {}}
-    # The single network trains on the first --graphs rows (grid, geometric,
-    # RMAT); the union network on every row, Erdos-Renyi included.
-    for name, fname, rows_used, er_every in (("single", "filter_mlp.f32", a.graphs, 0),
-                                             ("union", "filter_mlp_union.f32", total, 4 if a.union_er else 0)):
+    # The single network trains on the grid, geometric and RMAT rows (the
+    # first --graphs and the --large ones); the union network on the first
+    # --graphs and the Erdos-Renyi rows.
+    base = list(range(a.graphs))
+    er_rows = list(range(a.graphs, a.graphs + a.union_er))
+    large_rows = list(range(a.graphs + a.union_er, total))
+    for name, fname, rows, er_every in (("single", "filter_mlp.f32", base + large_rows, 0),
+                                        ("union", "filter_mlp_union.f32", base + er_rows, 4 if a.union_er else 0)):
         model = MODELS[name]
-        qs = [best_q(list(zip(R[i], BI[i], BR[i])), model) for i in range(rows_used)]
+        rows_used = len(rows)
+        qs = [best_q(list(zip(R[i], BI[i], BR[i])), model) for i in rows]
         Y = np.log2(np.array(qs, np.float32))
         p0 = emst.init_params(a.seed)
         t1 = time.time()
-        params, loss = emst.mlp_train(p0, X[:rows_used], Y, a.epochs, a.lr, a.seed)
+        params, loss = emst.mlp_train(p0, X[rows], Y, a.epochs, a.lr, a.seed)
         print(f"[train_filter] {name}: trained {a.epochs} epochs in {time.time()-t1:.1f}s, final mse {loss:.4f} "
               f"(label variance {float(np.var(Y)):.4f})", flush=True)
         params.astype(np.float32).tofile(os.path.join(a.out, fname))
         ratio, ev = evaluate(solver, params, model, a.holdout, a.seed, er_every)
         print(f"[train_filter] {name}: held-out geomean learned/plain modelled cost = {ratio:.3f}", flush=True)
         info["models"][name] = {"file": fname, "graphs": rows_used,
-                                "train_kinds": {k: kinds[:rows_used].count(k) for k in sorted(set(kinds[:rows_used]))},
+                                "train_kinds": {k: [kinds[i] for i in rows].count(k) for k in sorted(set(kinds[i] for i in rows))},
                                 **model.as_dict(), "final_mse": loss,
                                 "label_variance": float(np.var(Y)), "holdout_geomean_cost_ratio": ratio,
                                 "holdout": ev, "label_q_histogram": {str(q): int(sum(1 for x in qs if x == q))
