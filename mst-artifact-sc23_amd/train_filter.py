@@ -3,9 +3,10 @@ solves (needs a HIP device).
 
 1. Generates N seeded graphs: 2D grids, random geometric graphs and RMAT,
    2^12..2^20 vertices (log-uniform), mixed degrees and weight ranges, and
-   --large more of 2^21..2^22.5 vertices for the single-solve network (it
-   is applied to graphs far above 2^20 vertices, and without them it
-   extrapolated the quantile badly on geometric and Erdos-Renyi graphs).
+   optionally --large more of 2^21..2^22.5 vertices for the single-solve
+   network (48 of them helped geometric and Erdos-Renyi graphs far above
+   2^20 vertices but moved the 512x512 grid from the unfiltered path to a
+   slower filtered one, so the shipped network is trained without them).
    The union network also sees Erdos-Renyi graphs (--union-er of them):
    the batched workload mixes them in, and a network that never saw one
    chose q near 0.5 where about 0.2 is best.
@@ -170,7 +171,7 @@ def main():
     ap.add_argument("--out", default="gpurun_out/filter")
     ap.add_argument("--costs", default=None, help="reuse a filter_costs.npz instead of solving the graphs again")
     ap.add_argument("--union-er", type=int, default=667, help="Erdos-Renyi graphs added for the union network")
-    ap.add_argument("--large", type=int, default=48,
+    ap.add_argument("--large", type=int, default=0,
                     help="graphs of 2^21-2^22.5 vertices added for the single-solve network")
     a = ap.parse_args()
     os.makedirs(a.out, exist_ok=True)
