@@ -393,6 +393,25 @@ def run_ladder(a, emst):
             log(f"{name} contract: {ms:.3f} ms ({len(ts)} solves) same_weight={same_w}")
             if not same_w:
                 raise SystemExit(f"{name} contract: forest weight differs")
+        if a.ladder_rungs is None or "edgelist" in a.ladder_rungs.split(","):
+            # the second related-work baseline: the same (w, j) order, so the
+            # same inMST bytes
+            if ref is None:
+                s.solve(emst.FILTER_PLAIN if uf else emst.FILTER_OFF, T)
+                ref = s.fetch()
+            s.edgelist_mst(g)
+            ts, t0 = [], time.perf_counter()
+            while len(ts) < a.steps and (time.perf_counter() - t0 < a.ladder_budget or len(ts) < 1):
+                got_e, st_e = s.edgelist_mst(g)
+                ts.append(st_e.ms)
+            exact = bool(np.array_equal(got_e, ref))
+            ms = float(np.median(ts))
+            row["edgelist"] = {"ms": round(ms, 4), "medges_s": round(g.edges / ms / 1e3, 1), "solves": len(ts),
+                               "exact": exact, "iterations": int(st_e.iterations),
+                               "label_passes": int(st_e.label_passes)}
+            log(f"{name} edgelist: {ms:.3f} ms ({len(ts)} solves) exact={exact}")
+            if not exact:
+                raise SystemExit(f"{name} edgelist: inMST differs from the fused schedule")
         rows[name] = {"nodes": g.nodes, "edges": g.edges, "rungs": row}
     s.close()
     return {"metric": "de-optimization ladder: solve time per rung (device events, graph resident)",

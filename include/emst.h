@@ -285,11 +285,15 @@ void emst_free_graph(emst_graph_buf* g);                           /* ECLgraph.h
  *   emst_write_pbbs    PBBS WeightedEdgeArray text (PBBS_CPU/inputs/ecl2pbbs_genweights.cpp)
  *   emst_write_galois  Galois/Lonestar binary, version 1 (Lonestar_CPU/inputs/ecl2lsg_genweights.cpp)
  *   emst_write_uminho  BoruvkaUMinho edge list (UMinho_BOTH/inputs/ecl2UMinho_genweights.cpp)
+ *   emst_write_jucele  Jucele edge list, each undirected edge once (the layout
+ *                      Jucele_GPU/JuceleMST.cu:410-440 reads; its converter is
+ *                      not in the reference)
  * EMST_E_ARG without weights, EMST_E_OPEN / EMST_E_WRITE on file errors. */
 int emst_write_mtx(const char* path, const emst_graph* g);
 int emst_write_pbbs(const char* path, const emst_graph* g);
 int emst_write_galois(const char* path, const emst_graph* g);
 int emst_write_uminho(const char* path, const emst_graph* g);
+int emst_write_jucele(const char* path, const emst_graph* g);
 /* Weights for weightless inputs, ECL_MST.cu:410-419: eweight[j] =
  * |1 + ((i*k) % nodes)| with a wrapping 32-bit multiply.  Allocates
  * g->eweight (freed by emst_free_graph). */
@@ -323,6 +327,24 @@ typedef struct {
   uint64_t weight;        /* their total weight, as unsigned 32-bit values  */
 } emst_contract_stats;
 int emst_contract_mst(emst_ctx* ctx, const emst_graph* g, uint8_t* inMST, emst_contract_stats* stats);
+
+/* Edge-list Boruvka with components found by label propagation, the
+ * algorithm of the reference's Jucele GPU code (Jucele_GPU/JuceleMST.cu:
+ * 191-355 timed host loop, 456-656 kernels), run on the context's device for
+ * comparison with gpuMST.  Loads g (as emst_load), builds the list of
+ * upper-direction positions (nlist[j] > source) outside the timed loop, times
+ * the loop with device events and marks the forest in inMST[edges].  Ties go
+ * to the smaller position and weights compare as unsigned 32-bit values, as
+ * in gpuMST, so inMST is byte-identical to emst_gpu_mst's for the same graph.
+ * Any CSR that emst_validate accepts is allowed. */
+typedef struct {
+  double ms;              /* device time of the Boruvka loop                */
+  int64_t iterations;     /* iterations that joined edges                   */
+  int64_t label_passes;   /* label-propagation passes, all iterations       */
+  int64_t mst_edges;      /* marked forest edges                            */
+  uint64_t weight;        /* their total weight, as unsigned 32-bit values  */
+} emst_edgelist_stats;
+int emst_edgelist_mst(emst_ctx* ctx, const emst_graph* g, uint8_t* inMST, emst_edgelist_stats* stats);
 
 /* Library version string. */
 const char* emst_version(void);

@@ -30,8 +30,8 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-Wno-unused-function"]
 
 HOST_SOURCES = ["egr_io.cpp", "generators.cpp", "cpu_mst.cpp", "formats.cpp"]
-HIP_SOURCES = ["emst_solver.hip", "learned.hip", "batch.hip", "ladder.hip", "contract.hip"]
-CLI_SOURCES = ["ecl_mst_cli.cpp", "ecl_gen_cli.cpp", "ecl_convert_cli.cpp", "contract_mst_cli.cpp"]
+HIP_SOURCES = ["emst_solver.hip", "learned.hip", "batch.hip", "ladder.hip", "contract.hip", "edgelist.hip"]
+CLI_SOURCES = ["ecl_mst_cli.cpp", "ecl_gen_cli.cpp", "ecl_convert_cli.cpp", "contract_mst_cli.cpp", "edgelist_mst_cli.cpp"]
 
 
 def _run(cmd):
@@ -63,7 +63,7 @@ def build(jobs=8):
     tmp = lib + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-pthread"])
     os.replace(tmp, lib)
-    for name, obj in zip(("ecl_mst", "ecl_gen", "ecl_convert", "contract_mst"), cli_objs):
+    for name, obj in zip(("ecl_mst", "ecl_gen", "ecl_convert", "contract_mst", "edgelist_mst"), cli_objs):
         _run([CXX, "-o", os.path.join(BIN, name), obj, f"-L{LIB}", "-lemst", "-Wl,-rpath,$ORIGIN/../lib", "-pthread"])
     return lib
 

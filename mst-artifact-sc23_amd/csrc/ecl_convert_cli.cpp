@@ -10,7 +10,7 @@
 int main(int argc, char* argv[])
 {
   if (argc != 4) {
-    std::fprintf(stderr, "USAGE: %s <input.egr> <mtx|pbbs|lsg|uminho|egr> <output>\n", argv[0]);
+    std::fprintf(stderr, "USAGE: %s <input.egr> <mtx|pbbs|lsg|uminho|jucele|egr> <output>\n", argv[0]);
     return 1;
   }
   emst_graph_buf g{};
@@ -36,6 +36,7 @@ int main(int argc, char* argv[])
   else if (fmt == "pbbs") rc = emst_write_pbbs(argv[3], &v);
   else if (fmt == "lsg") rc = emst_write_galois(argv[3], &v);
   else if (fmt == "uminho") rc = emst_write_uminho(argv[3], &v);
+  else if (fmt == "jucele") rc = emst_write_jucele(argv[3], &v);
   else if (fmt == "egr") rc = emst_write_egr(argv[3], &v);
   else {
     std::fprintf(stderr, "ERROR: unknown format %s\n", fmt.c_str());

@@ -134,4 +134,24 @@ int emst_write_uminho(const char* path, const emst_graph* g)
   return o.close();
 }
 
+// Jucele edge list (the layout JuceleMST.cu:410-440 reads): the vertex
+// count and the edge count on two lines, then each undirected edge once, as
+// "src dst weight", here the upper-direction CSR positions (dst > src) in
+// order, the edges gpuMST and emst_edgelist_mst work on.
+int emst_write_jucele(const char* path, const emst_graph* g)
+{
+  if (!path || !valid(g)) return EMST_E_ARG;
+  int64_t up = 0;
+  for (int32_t v = 0; v < g->nodes; v++)
+    for (int32_t j = g->nindex[v]; j < g->nindex[v + 1]; j++) up += g->nlist[j] > v;
+  FILE* f = std::fopen(path, "wt");
+  if (!f) return EMST_E_OPEN;
+  TextOut o(f);
+  o.printf_("%d\n%" PRId64 "\n", g->nodes, up);
+  for (int32_t v = 0; v < g->nodes; v++)
+    for (int32_t j = g->nindex[v]; j < g->nindex[v + 1]; j++)
+      if (g->nlist[j] > v) o.printf_("%d %d %d\n", v, g->nlist[j], g->eweight[j]);
+  return o.close();
+}
+
 }  // extern "C"

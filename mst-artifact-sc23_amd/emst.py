@@ -130,7 +130,7 @@ _solve_device = _sig("emst_solve_device", C.c_int, C.c_void_p, C.c_int32, C.c_in
 _read_egr = _sig("emst_read_egr", C.c_int, C.c_char_p, C.POINTER(_GraphBuf))
 _write_egr = _sig("emst_write_egr", C.c_int, C.c_char_p, C.POINTER(_Graph))
 _writers = {fmt: _sig(f"emst_write_{fmt}", C.c_int, C.c_char_p, C.POINTER(_Graph))
-            for fmt in ("mtx", "pbbs", "galois", "uminho")}
+            for fmt in ("mtx", "pbbs", "galois", "uminho", "jucele")}
 _free_graph = _sig("emst_free_graph", None, C.POINTER(_GraphBuf))
 _synth_weights = _sig("emst_synth_weights", C.c_int, C.POINTER(_GraphBuf))
 _mst_weight = _sig("emst_mst_weight", C.c_uint64, C.POINTER(_Graph), _pu8)
@@ -179,6 +179,14 @@ class ContractStats(C.Structure):
 
 
 _contract = _sig("emst_contract_mst", C.c_int, C.c_void_p, C.POINTER(_Graph), _pu8, C.POINTER(ContractStats))
+
+
+class EdgelistStats(C.Structure):
+    _fields_ = [("ms", C.c_double), ("iterations", C.c_int64), ("label_passes", C.c_int64),
+                ("mst_edges", C.c_int64), ("weight", C.c_uint64)]
+
+
+_edgelist = _sig("emst_edgelist_mst", C.c_int, C.c_void_p, C.POINTER(_Graph), _pu8, C.POINTER(EdgelistStats))
 _autotune = _sig("emst_autotune", C.c_int, C.c_void_p, C.c_int, C.c_int32, C.c_int32, C.POINTER(Config),
                  C.POINTER(C.c_double))
 
@@ -246,7 +254,7 @@ def write_egr(path, g):
 
 
 def write_format(path, g, fmt):
-    """One of the reference's baseline formats: "mtx", "pbbs", "galois" or "uminho"."""
+    """One of the reference's baseline formats: "mtx", "pbbs", "galois", "uminho" or "jucele"."""
     _check(_writers[fmt](os.fsencode(path), C.byref(g.view())), path)
 
 
@@ -414,6 +422,16 @@ class Solver:
         out = np.zeros(max(g.edges, 1), np.uint8)
         st = ContractStats()
         self._err(_contract(self._h, C.byref(g.view()), out.ctypes.data_as(_pu8), C.byref(st)))
+        self.graph = g
+        return out[:g.edges], st
+
+    def edgelist_mst(self, g):
+        """The second related-work baseline (edge-list Boruvka with label
+        propagation, the Jucele GPU algorithm): (inMST, EdgelistStats).
+        inMST equals gpu_mst's bytes for the same graph."""
+        out = np.zeros(max(g.edges, 1), np.uint8)
+        st = EdgelistStats()
+        self._err(_edgelist(self._h, C.byref(g.view()), out.ctypes.data_as(_pu8), C.byref(st)))
         self.graph = g
         return out[:g.edges], st
 

@@ -10,8 +10,8 @@ the .egr file's header, not from a fixed list.
   python tables.py run <out_dir> <graph.egr>...    runs bin/ecl_mst on every
       graph, once with the reference's threshold rule (ecl_mst_out.csv) and
       once with the trained network (ecl_mst_learned_out.csv), and
-      bin/contract_mst, the related-work baseline (UMinho_gpu_out.csv), then
-      writes the tables
+      the related-work baselines bin/contract_mst (UMinho_gpu_out.csv) and
+      bin/edgelist_mst (Jucele_gpu_out.csv), then writes the tables
   python tables.py table <out_dir> <capture.csv>...    tables from captures
       made earlier (each capture becomes one column)
   python tables.py deopt <out_dir> <graph.egr>...    the de-optimization
@@ -32,6 +32,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CLI = os.path.join(HERE, "bin", "ecl_mst")
 # the related-work baseline (Boruvka with graph contraction, UMinho GPU)
 CONTRACT_CLI = os.path.join(HERE, "bin", "contract_mst")
+# the second one (edge-list Boruvka with label propagation, Jucele GPU)
+EDGELIST_CLI = os.path.join(HERE, "bin", "edgelist_mst")
 RUNS = 9
 # Rungs of the ladder by the reference's binary suffixes
 # (Deoptimizations/run_all_ECL-MST_deopts.sh:7-15), preceded by the fused
@@ -111,7 +113,7 @@ def run(out_dir, graphs):
     os.makedirs(out_dir, exist_ok=True)
     columns = []
     for col, env, cli in (("ecl_mst", {}, CLI), ("ecl_mst_learned", {"EMST_FILTER": "learned"}, CLI),
-                          ("UMinho_gpu", {}, CONTRACT_CLI)):
+                          ("UMinho_gpu", {}, CONTRACT_CLI), ("Jucele_gpu", {}, EDGELIST_CLI)):
         text = capture(graphs, env, cli)
         with open(os.path.join(out_dir, f"{col}_out.csv"), "w") as f:
             f.write(text)

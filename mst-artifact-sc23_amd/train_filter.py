@@ -168,7 +168,7 @@ def main():
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--epochs", type=int, default=4000)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--out", default="gpurun_out/filter")
+    ap.add_argument("--out", default="filter_train", help="directory for the trained networks and labels")
     ap.add_argument("--costs", default=None, help="reuse a filter_costs.npz instead of solving the graphs again")
     ap.add_argument("--union-er", type=int, default=667, help="Erdos-Renyi graphs added for the union network")
     ap.add_argument("--large", type=int, default=0,

@@ -296,6 +296,13 @@ def test_format_writers(tmp_path):
     lines = p.read_text().splitlines()
     assert lines[0] == f"{g.nodes} {g.edges}"
     assert lines[1:] == [f"{s} {d} {w}" for s, d, w in zip(src, g.nlist, g.eweight)]
+    # Jucele edge list: counts on two lines, each undirected edge once
+    p = tmp_path / "g.jmst"
+    emst.write_format(str(p), g, "jucele")
+    lines = p.read_text().splitlines()
+    up = [(s, d, w) for s, d, w in zip(src, g.nlist, g.eweight) if d > s]
+    assert lines[:2] == [f"{g.nodes}", f"{len(up)}"]
+    assert lines[2:] == [f"{s} {d} {w}" for s, d, w in up]
     # Galois/Lonestar binary v1
     p = tmp_path / "g.gr"
     emst.write_format(str(p), g, "galois")

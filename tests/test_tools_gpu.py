@@ -25,13 +25,13 @@ def test_cli_modes_and_tables(tmp_path):
     assert r.returncode == 0, r.stderr
     print("step: tables.run", flush=True)
     runtime, thru = tables.run(str(tmp_path), [str(egr), str(grid)])
-    for col in ("ecl_mst", "ecl_mst_learned", "UMinho_gpu"):
+    for col in ("ecl_mst", "ecl_mst_learned", "UMinho_gpu", "Jucele_gpu"):
         cap = (tmp_path / f"{col}_out.csv").read_text()
         assert "ERROR" not in cap and cap.count("input:") == 2
         for p in (str(egr), str(grid)):
             assert runtime[col][p] > 0 and thru[col][p] > 0
     lines = (tmp_path / "comparison_throughputs.csv").read_text().splitlines()
-    assert lines[0] == ",ecl_mst, ecl_mst_learned, UMinho_gpu, "
+    assert lines[0] == ",ecl_mst, ecl_mst_learned, UMinho_gpu, Jucele_gpu, "
     assert [ln.split(",")[0] for ln in lines[1:]] == ["rmat12", "grid"]
 
 
