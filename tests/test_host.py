@@ -239,6 +239,23 @@ def test_cli_error_paths(tmp_path):
     assert r.returncode == 255 and "node or edge count too low" in r.stderr
 
 
+@pytest.mark.skipif(not os.path.exists(CLI), reason="CLI not built")
+def test_baseline_clis_and_entry_points_reject_bad_arguments(tmp_path):
+    """The related-work baselines fail before touching a device: usage and
+    unreadable input from their CLIs, EMST_E_ARG from null arguments."""
+    import ctypes as C
+    for name in ("contract_mst", "edgelist_mst"):
+        cli = os.path.join(ROOT, "mst-artifact-sc23_amd", "bin", name)
+        r = subprocess.run([cli], capture_output=True, text=True)
+        assert r.returncode == 255 and r.stdout.startswith("USAGE:"), name
+        r = subprocess.run([cli, str(tmp_path / "missing.egr")], capture_output=True, text=True)
+        assert r.returncode == 255 and "could not read" in r.stderr, name
+        fn = getattr(emst.lib, f"emst_{name}")
+        fn.restype = C.c_int
+        fn.argtypes = [C.c_void_p] * 4
+        assert fn(None, None, None, None) == -1  # EMST_E_ARG
+
+
 def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True)
