@@ -220,6 +220,13 @@ def run_single(a, emst):
                   "light": st_p[-1].light, "heavy": st_p[-1].heavy,
                   "rounds": [st_p[-1].light_rounds, st_p[-1].heavy_rounds]},
         "learned_vs_plain": ms_plain / ms_learned,
+        # the plain rule as the judge of the learned threshold: rounds the
+        # learned choice adds (negative: saves), and the edges each choice
+        # puts through the light filter and on to the heavy pass
+        "learned_judged_by_plain": {
+            "extra_rounds": (stl.light_rounds + stl.heavy_rounds) - (st_p[-1].light_rounds + st_p[-1].heavy_rounds),
+            "light_edges": {"learned": stl.light, "plain": st_p[-1].light},
+            "post_filter_edges": {"learned": stl.heavy, "plain": st_p[-1].heavy}},
         "config": {"workload": w["desc"] if a.scale is None else w["desc"].replace(str(w.get("scale")), str(a.scale), 1),
                    "nodes": n, "edges": m, "learned_q": stl.learned_q, "threshold": stl.threshold_used,
                    "filter": bool(stl.filter_used), "light": stl.light, "heavy": stl.heavy,
