@@ -15,6 +15,10 @@ The metric is directed CSR edges per second (generate_compare_tables.py:
 n log-uniform in [1e3, 1e6], generator uniform over grid / geometric deg 6 /
 RMAT ef 8 / Erdos-Renyi deg 10, weights in [1, 2^20), seed 4) and reports
 graphs/s with p50/p99 per-graph latency against the per-graph loop.
+--workload ladder times the de-optimization ladder and the related-work
+baselines; --workload shard runs the edge-sharded single-graph scheme shard by
+shard on this GPU (--shards, --shard-workload) and checks it against one
+whole-graph solve.
 
 Prints one JSON line.  --gpus above 1 is refused: the solver runs on one
 GPU.
@@ -425,6 +429,57 @@ def run_ladder(a, emst):
             "unit": "ms", "n_gpus": 1, "data": "synthetic", "weights": "ECL_MST.cu:410-419", "inputs": rows}
 
 
+XGMI_GBS = 7 * 153.0  # seven xGMI links into one MI355X, GB/s each way
+
+
+def run_shard(a, emst):
+    """The single-graph multi-GPU scheme at one GPU's real shape, shard by
+    shard on this device: the upper-direction edges of --shard-workload are
+    cut into --shards contiguous ranges; each range is loaded and solved as
+    one rank would (device-resident, learned threshold, median of --steps),
+    the positions its forest marks are gathered, and the gathered set is
+    solved once more.  The result must equal one solve over the whole graph.
+    The implied k-GPU job time is the slowest shard, plus the gather of the
+    shard forests (16-byte records at the xGMI rate into one GPU), plus the
+    merge solve; nothing here runs on more than one GPU."""
+    import shard
+    w = WORKLOADS[a.shard_workload]
+    g = make_graph(emst, w, a.scale)
+    log(f"shard input {a.shard_workload}: n={g.nodes} m={g.edges} k={a.shards}")
+    s = emst.Solver(0)
+    s.set_filter_model(np.fromfile(os.path.join(ROOT, "mst-artifact-sc23_amd", "filter_mlp.f32"), np.float32))
+
+    def timed(h):
+        s.load(h)
+        _, st = timed_steps(s, emst.FILTER_LEARNED, 0, a.steps, a.warmup)
+        return s.fetch(), statistics.median(x.ms for x in st)
+    full, full_ms = timed(g)
+    times = []
+
+    def solve(h):
+        mark, ms = timed(h)
+        times.append(ms)
+        return mark
+    got, info = shard.solve_sharded(g, a.shards, solve)
+    exact = bool(np.array_equal(got, full))
+    shard_ms, merge_ms = times[:-1], times[-1]
+    gather_ms = 16 * info["gathered"] / (XGMI_GBS * 1e9) * 1e3
+    implied = max(shard_ms) + gather_ms + merge_ms
+    log(f"shards {[round(t, 3) for t in shard_ms]} ms, merge {merge_ms:.3f} ms, gathered {info['gathered']}, "
+        f"whole graph {full_ms:.3f} ms, exact={exact}")
+    s.close()
+    if not exact:
+        raise SystemExit("sharded forest differs from the whole-graph forest")
+    return {"metric": "edge-sharded solve, shard by shard on one GPU (device-resident, learned threshold)",
+            "unit": "ms", "n_gpus": 1, "data": "synthetic", "workload": w["desc"], "nodes": g.nodes,
+            "edges": g.edges, "shards": a.shards, "shard_upper_edges": info["shard_edges"],
+            "shard_forest_edges": info["shard_forest_edges"], "gathered": info["gathered"],
+            "shard_ms": [round(t, 4) for t in shard_ms], "merge_ms": round(merge_ms, 4),
+            "gather_model_ms": round(gather_ms, 4), "implied_k_gpu_ms": round(implied, 4),
+            "whole_graph_ms": round(full_ms, 4), "exact": exact,
+            "note": "implied_k_gpu_ms = slowest shard + modelled gather + merge; no run used more than one GPU"}
+
+
 def run_copy(a):
     """Calibration for HBM traffic counters: the HIP runtime's device-to-
     device copy of a known size (read 1 GiB, write 1 GiB per launch).  Run
@@ -456,7 +511,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=9)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="rmat24", choices=sorted(WORKLOADS) + ["batch", "copy", "ladder"])
+    ap.add_argument("--workload", default="rmat24", choices=sorted(WORKLOADS) + ["batch", "copy", "ladder", "shard"])
     ap.add_argument("--scale", type=int, default=None, help="override the RMAT scale (testing only)")
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -470,6 +525,8 @@ def main():
     ap.add_argument("--ladder-graphs", default=",".join(LADDER_GRAPHS))
     ap.add_argument("--ladder-budget", type=float, default=5.0, help="seconds of solves per rung before stopping")
     ap.add_argument("--ladder-rungs", default=None, help="comma list of rung names (default: every rung)")
+    ap.add_argument("--shards", type=int, default=8, help="edge shards of --workload shard")
+    ap.add_argument("--shard-workload", default="rmat24", choices=sorted(WORKLOADS))
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "
@@ -481,6 +538,8 @@ def main():
     import emst
     if a.workload == "ladder":
         out = run_ladder(a, emst)
+    elif a.workload == "shard":
+        out = run_shard(a, emst)
     else:
         out = run_batch(a, emst) if a.workload == "batch" else run_single(a, emst)
     print(json.dumps(out))
