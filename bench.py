@@ -436,7 +436,8 @@ def run_shard(a, emst):
     """The single-graph multi-GPU scheme at one GPU's real shape, shard by
     shard on this device: the upper-direction edges of --shard-workload are
     cut into --shards contiguous ranges; each range is loaded and solved as
-    one rank would (device-resident, learned threshold, median of --steps),
+    one rank would (device-resident, median of --steps, with the threshold
+    rule of --shard-filter),
     the positions its forest marks are gathered, and the gathered set is
     solved once more.  The result must equal one solve over the whole graph.
     The implied k-GPU job time is the slowest shard, plus the gather of the
@@ -451,7 +452,12 @@ def run_shard(a, emst):
 
     def timed(h):
         s.load(h)
-        _, st = timed_steps(s, emst.FILTER_LEARNED, 0, a.steps, a.warmup)
+        if a.shard_filter == "learned":
+            mode, thr = emst.FILTER_LEARNED, 0
+        else:
+            uf, thr = emst.plain_threshold(h)
+            mode = emst.FILTER_PLAIN if uf else emst.FILTER_OFF
+        _, st = timed_steps(s, mode, thr, a.steps, a.warmup)
         return s.fetch(), statistics.median(x.ms for x in st)
     full, full_ms = timed(g)
     times = []
@@ -470,7 +476,7 @@ def run_shard(a, emst):
     s.close()
     if not exact:
         raise SystemExit("sharded forest differs from the whole-graph forest")
-    return {"metric": "edge-sharded solve, shard by shard on one GPU (device-resident, learned threshold)",
+    return {"metric": f"edge-sharded solve, shard by shard on one GPU (device-resident, {a.shard_filter} threshold)",
             "unit": "ms", "n_gpus": 1, "data": "synthetic", "workload": w["desc"], "nodes": g.nodes,
             "edges": g.edges, "shards": a.shards, "shard_upper_edges": info["shard_edges"],
             "shard_forest_edges": info["shard_forest_edges"], "gathered": info["gathered"],
@@ -527,6 +533,7 @@ def main():
     ap.add_argument("--ladder-rungs", default=None, help="comma list of rung names (default: every rung)")
     ap.add_argument("--shards", type=int, default=8, help="edge shards of --workload shard")
     ap.add_argument("--shard-workload", default="rmat24", choices=sorted(WORKLOADS))
+    ap.add_argument("--shard-filter", default="learned", choices=["learned", "plain"])
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "
