@@ -4,12 +4,13 @@ C1 grid 512x512, C2 RMAT-24 and C3 RGG-2^24: the solve's counts (records,
 rounds, worklist sizes entering and leaving the rounds, weight) against the
 oracle's round schedule committed in tests/golden/schedules.json (made by
 make_schedules.py); C2 and C3 also in every filter mode against the
-oracle's serial Kruskal, bit for bit.  C5 RMAT-26 with weights in [1, 64]
-(heavy ties, a forest, ~2e9 directed edges): the forest is unique under
-the (w, j) order, so the unfiltered, plain and learned runs must give
-identical bytes; the oracle's union-find then checks that it is a spanning
-forest, and the weights agree with the OpenMP Boruvka restatement.  Set EMST_SKIP_RMAT26=1
-to leave C5 out."""
+oracle's serial Kruskal, bit for bit, and so are the edge-list baseline
+and the 4-shard solve (the contraction baseline by forest weight).
+C5 RMAT-26 with weights in [1, 64] (heavy ties, a forest, ~2e9 directed
+edges): the forest is unique under the (w, j) order, so the unfiltered,
+plain and learned runs must give identical bytes; the oracle's union-find
+then checks that it is a spanning forest, and the weights agree with the
+OpenMP Boruvka restatement.  Set EMST_SKIP_RMAT26=1 to leave C5 out."""
 import json
 import os
 import time
@@ -19,6 +20,7 @@ import pytest
 
 import emst
 import oracle_lib as O
+import shard
 from golden.make_schedules import digest
 
 pytestmark = [pytest.mark.gpu, pytest.mark.slow]
@@ -90,12 +92,23 @@ def test_c2_c3_against_kruskal(cfg):
     s = _solver()
     res = _modes(s, g)
     check_schedules(s, g, cfg)
+    # the related-work baselines at full size: the edge-list Boruvka must
+    # give the same bytes, the contraction Boruvka the same forest weight
+    el, est = s.edgelist_mst(g)
+    ct, cst = s.contract_mst(g)
+    print(f"  edgelist {est.ms:.2f} ms ({est.iterations} iterations), contract {cst.ms:.2f} ms", flush=True)
+    # the edge-sharded scheme (4 shards, learned threshold), shard by shard
+    sharded, info = shard.solve_sharded(g, 4, lambda h: s.gpu_mst(h, emst.FILTER_LEARNED, 0)[0])
+    print(f"  sharded: gathered {info['gathered']}", flush=True)
     s.close()
     ref = O.kruskal(g)
     for name, (got, st) in res.items():
         assert np.array_equal(got, ref), name
     rc, cw, gw, oc, og = O.verify(g, ref, res["learned"][0])
     assert rc == 0 and cw == gw and oc == og == 0
+    assert np.array_equal(el, ref) and est.weight == cw
+    assert cst.weight == cw and cst.mst_edges == int(ref.sum())
+    assert np.array_equal(sharded, ref)
 
 
 @pytest.mark.skipif(os.environ.get("EMST_SKIP_RMAT26") == "1", reason="EMST_SKIP_RMAT26=1")
