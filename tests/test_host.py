@@ -256,6 +256,40 @@ def test_baseline_clis_and_entry_points_reject_bad_arguments(tmp_path):
         assert fn(None, None, None, None) == -1  # EMST_E_ARG
 
 
+NULL_CALLS = {
+    "emst_set_config": 2, "emst_get_config": 2, "emst_gpu_mst": (None, None, 0, 0, None, None),
+    "emst_load": 2, "emst_solve": (None, 0, 0, None), "emst_fetch": 2, "emst_autotune": (None, 0, 0, 1, None, None),
+    "emst_solve_device": (None, 1, 0, None, None, None, 0, 0, None, None, None), "emst_fetch_parent": 2,
+    "emst_batch_create": (None, 1, None, 0, None), "emst_batch_info": 4, "emst_batch_solve": (None, 0, None, 0, None),
+    "emst_batch_fetch": (None, 0, None), "emst_batch_thresholds": 3, "emst_batch_set_filter_model": 2,
+    "emst_set_filter_model": 2, "emst_learned_threshold": 6, "emst_contract_mst": 4, "emst_edgelist_mst": 4,
+}
+
+
+def test_every_context_entry_point_rejects_null():
+    """Each entry point that takes a context or batch returns EMST_E_ARG for
+    a null one (no device is touched), destroy accepts null, and
+    last_error(NULL) answers; run in a child process so a crash is a
+    failure, not the end of the suite."""
+    code = ["import ctypes as C, sys, json", f"sys.path.insert(0, {os.path.join(ROOT, 'mst-artifact-sc23_amd')!r})",
+            "import emst", f"calls = {NULL_CALLS!r}", "out = {}",
+            "for name, args in calls.items():",
+            "    args = (None,) * args if isinstance(args, int) else args",
+            "    f = getattr(emst.lib, name); f.restype = C.c_int",
+            "    f.argtypes = [C.c_void_p if a is None else C.c_int for a in args]",
+            "    out[name] = f(*args)",
+            "for name in ('emst_destroy', 'emst_batch_destroy'):",
+            "    f = getattr(emst.lib, name); f.restype = None; f.argtypes = [C.c_void_p]; f(None)",
+            "emst.lib.emst_last_error.restype = C.c_char_p; emst.lib.emst_last_error.argtypes = [C.c_void_p]",
+            "out['last_error'] = emst.lib.emst_last_error(None).decode()",
+            "print(json.dumps(out))"]
+    r = subprocess.run([sys.executable, "-c", "\n".join(code)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out.pop("last_error")
+    assert out == {name: -1 for name in NULL_CALLS}  # EMST_E_ARG
+
+
 def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True)
