@@ -318,7 +318,9 @@ int emst_verify(const emst_graph* g, const uint8_t* cpu, const uint8_t* gpu, uin
  * under equal weights its edges can differ from gpuMST's, and either
  * direction's position of an edge may be marked.  Weights compare as
  * unsigned 32-bit values.  Needs a symmetric CSR, as the reference's inputs
- * are: one-way edges can close a cycle of picks, reported as EMST_E_LIMIT. */
+ * are (every edge (v, u, w) has a reverse (u, v, w), with the same
+ * multiplicity): one-way edges can close a cycle of picks, so any other input
+ * is refused with EMST_E_GRAPH before the device is used. */
 typedef struct {
   double ms;              /* device time of the contraction loop            */
   int32_t iterations;     /* contraction iterations                         */

@@ -186,12 +186,16 @@ int emst_batch_create(emst_ctx* ctx, int32_t ngraphs, const emst_graph* graphs, 
       rc = ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of a batch arena failed");
       break;
     }
-    ok = hipMemcpy(c.d_nindex, nindex.data(), ((size_t)c.nodes + 1) * 4, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(c.d_nlist, nlist.data(), (size_t)c.edges * 4, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(c.d_eweight, ew.data(), (size_t)c.edges * 4, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(c.d_vgid, vgid.data(), (size_t)c.nodes * 4, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(c.d_voff, c.voff.data(), (size_t)(G + 1) * 8, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(c.d_eoff, c.eoff.data(), (size_t)(G + 1) * 8, hipMemcpyHostToDevice) == hipSuccess;
+    // the arena's kernels run on the context's non-blocking stream: copy on
+    // that stream and wait before the host staging vectors go out of scope
+    hipStream_t cs = ctx->stream;
+    ok = hipMemcpyAsync(c.d_nindex, nindex.data(), ((size_t)c.nodes + 1) * 4, hipMemcpyHostToDevice, cs) == hipSuccess &&
+         hipMemcpyAsync(c.d_nlist, nlist.data(), (size_t)c.edges * 4, hipMemcpyHostToDevice, cs) == hipSuccess &&
+         hipMemcpyAsync(c.d_eweight, ew.data(), (size_t)c.edges * 4, hipMemcpyHostToDevice, cs) == hipSuccess &&
+         hipMemcpyAsync(c.d_vgid, vgid.data(), (size_t)c.nodes * 4, hipMemcpyHostToDevice, cs) == hipSuccess &&
+         hipMemcpyAsync(c.d_voff, c.voff.data(), (size_t)(G + 1) * 8, hipMemcpyHostToDevice, cs) == hipSuccess &&
+         hipMemcpyAsync(c.d_eoff, c.eoff.data(), (size_t)(G + 1) * 8, hipMemcpyHostToDevice, cs) == hipSuccess &&
+         hipStreamSynchronize(cs) == hipSuccess;
     if (!ok) rc = ctx_fail(ctx, EMST_E_HIP, "batch arena upload failed");
   }
   if (!rc) {
@@ -204,7 +208,7 @@ int emst_batch_create(emst_ctx* ctx, int32_t ngraphs, const emst_graph* graphs, 
       me = std::max(me, c.edges);
       mu = std::max(mu, c.m_up);
     }
-    rc = ensure_buffers(ctx, mn, me, mu, 256);
+    rc = ensure_buffers(ctx, mn, me, mu);
   }
   if (!rc && (hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess))
     rc = ctx_fail(ctx, EMST_E_HIP, "hipEventCreate failed");
@@ -243,7 +247,8 @@ int emst_batch_set_filter_model(emst_batch* b, const float* params)
   if (hipSetDevice(ctx->device) != hipSuccess) return ctx_fail(ctx, EMST_E_HIP, "hipSetDevice failed");
   if (!b->d_mlp16 && hipMalloc((void**)&b->d_mlp16, kParams * 2) != hipSuccess)
     return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of the batch network failed");
-  if (hipMemcpy(b->d_mlp16, h.data(), kParams * 2, hipMemcpyHostToDevice) != hipSuccess)
+  if (hipMemcpyAsync(b->d_mlp16, h.data(), kParams * 2, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess)
     return ctx_fail(ctx, EMST_E_HIP, "upload of the batch network failed");
   b->have_model = 1;
   return EMST_OK;

@@ -41,6 +41,7 @@
 
 #include "emst_internal.hpp"
 #include "emst_kernels.hpp"
+#include "host_util.hpp"
 
 using namespace emst;
 
@@ -442,11 +443,55 @@ struct Buf {
 
 }  // namespace
 
+// True when every directed edge (v, u, w) has a reverse (u, v, w), with the
+// same multiplicity.  Each list is copied as (neighbour, weight) keys and
+// sorted; then, per run of equal keys in v's list, the run of (v, w) in u's
+// list must have the same length.  O(m log d) host work, outside the timed
+// loop.
+static bool csr_symmetric(const emst_graph* g)
+{
+  const int64_t n = g->nodes;
+  std::vector<uint64_t> key((size_t)g->edges);
+  emst::parallel_for(n, [&](int64_t b, int64_t e, int) {
+    for (int64_t v = b; v < e; v++) {
+      const int32_t lo = g->nindex[v], hi = g->nindex[v + 1];
+      for (int32_t j = lo; j < hi; j++)
+        key[j] = ((uint64_t)(uint32_t)g->nlist[j] << 32) | (uint32_t)g->eweight[j];
+      std::sort(key.begin() + lo, key.begin() + hi);
+    }
+  });
+  std::vector<char> bad((size_t)emst::host_threads(), 0);
+  emst::parallel_for(n, [&](int64_t b, int64_t e, int t) {
+    for (int64_t v = b; v < e && !bad[t]; v++) {
+      const int32_t lo = g->nindex[v], hi = g->nindex[v + 1];
+      for (int32_t j = lo; j < hi;) {
+        const uint64_t k = key[j];
+        int32_t r = j + 1;
+        while (r < hi && key[r] == k) r++;
+        const int64_t u = (int64_t)(k >> 32);
+        const uint64_t back = ((uint64_t)(uint32_t)v << 32) | (uint32_t)k;
+        const auto rb = key.begin() + g->nindex[u], re = key.begin() + g->nindex[u + 1];
+        const auto range = std::equal_range(rb, re, back);
+        if (range.second - range.first != r - j) {
+          bad[t] = 1;
+          break;
+        }
+        j = r;
+      }
+    }
+  }, (int)bad.size());
+  for (char c : bad)
+    if (c) return false;
+  return true;
+}
+
 extern "C" int emst_contract_mst(emst_ctx* ctx, const emst_graph* g, uint8_t* inMST, emst_contract_stats* stats)
 {
   if (!ctx || !g || !inMST) return EMST_E_ARG;
   int rc = emst_load(ctx, g);
   if (rc) return rc;
+  // one-way edges can close a cycle of picks: such inputs are refused
+  if (!csr_symmetric(g)) return ctx_fail(ctx, EMST_E_GRAPH, "the contraction baseline needs a symmetric CSR");
   const int n0 = g->nodes, m0 = g->edges;
   hipStream_t s = ctx->stream;
   auto fail = [&](hipError_t e, const char* what) {
@@ -517,7 +562,10 @@ extern "C" int emst_contract_mst(emst_ctx* ctx, const emst_graph* g, uint8_t* in
           hc.push_back(make_int4(i, c, (int)std::min<long long>((long long)c + kChunk, hb[i].z), hb[i].x));
       if (hc.size() > chmax) return ctx_fail(ctx, EMST_E_LIMIT, "chunk table overflow");
       nchunks = (int)hc.size();
-      CCHK(hipMemcpy(b_chunks.p, hc.data(), hc.size() * sizeof(int4), hipMemcpyHostToDevice));
+      // on the solve's stream (non-blocking, so not ordered with the null
+      // stream); hc must outlive the copy, hence the synchronisation
+      CCHK(hipMemcpyAsync(b_chunks.p, hc.data(), hc.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+      CCHK(hipStreamSynchronize(s));
       CCHK(hipMemsetAsync(b_bigkey.p, 0xff, (size_t)nb * 8, s));
       CCHK(hipMemsetAsync(b_bigpick.p, 0xff, (size_t)nb * 8, s));
       hipLaunchKernelGGL(k_cbig_key, dim3(nchunks), dim3(kCBlock), 0, s, (const int4*)b_chunks.p, dst, wt,

@@ -31,10 +31,13 @@ struct emst_ctx {
   unsigned* d_vclass = nullptr;
   size_t cap_nodes = 0;
   int4* d_wl[2] = {nullptr, nullptr};
-  emst::u64* d_cand = nullptr;        // first-round candidate bitmasks
-  size_t cap_wl = 0;
+  size_t cap_wl[2] = {0, 0};
+  emst::u64* d_cand = nullptr;        // first-round candidate bitmasks, joined-record masks
+  size_t cap_cand = 0;
   int* d_gsrc = nullptr;
   size_t cap_gsrc = 0;
+  int* d_segcnt = nullptr;            // records per initialisation segment
+  size_t cap_seg = 0;
   int* d_cnt = nullptr;
   long long* d_stats = nullptr;
   int* h_flag = nullptr;
@@ -52,6 +55,7 @@ struct emst_ctx {
   size_t cap_lad_cnt = 0;
   hipEvent_t ev[8] = {};
   int32_t last_nodes = 0;
+  const int* last_parent = nullptr;   // parent array of the last solve (context or scratch)
   // learned threshold
   unsigned short* d_mlp16 = nullptr;  // bf16 parameters
   int have_model = 0;
@@ -65,6 +69,31 @@ struct emst_ctx {
 };
 
 namespace emst {
+
+// Device work buffers of one solve: the context's own (grown on demand) or
+// carved from the library's static scratch array.
+struct WorkBufs {
+  int* parent = nullptr;        // nodes
+  u64* minv = nullptr;          // nodes
+  unsigned* vclass = nullptr;   // two bits per vertex
+  int* gsrc = nullptr;          // start vertex of every initialisation task
+  int4* wl[2] = {nullptr, nullptr};
+  u64* cand = nullptr;          // two bitmasks of cand_words words
+  int* cnt = nullptr;           // [list 0 size, list 1 size, marked component, -]
+  long long* stats = nullptr;   // ST_COUNT counters
+  int* segcnt = nullptr;        // records per initialisation segment
+};
+
+// Element counts of the work buffers for a solve of the given size under a
+// configuration.  The initialisation kernels run init_grid workgroups, each
+// appending to its own segment of seg_cap records at the start of list 0.
+struct WorkSizes {
+  long long nodes = 0, vclass_words = 0, gsrc = 0, wl[2] = {0, 0}, cand_words = 0, segs = 0;
+  int init_grid = 0;
+  long long seg_cap = 0;
+};
+
+WorkSizes work_sizes(int cus, const emst_config& cfg, int32_t nodes, int32_t edges, long long wl_cap);
 
 // Per-vertex thresholds for a union of graphs solved together: the
 // threshold of vertex v's graph is gthr[vgid[v]].  Both NULL for one graph.
@@ -87,12 +116,17 @@ struct FixedSchedule {
 
 int ctx_fail(emst_ctx* c, int code, const std::string& what);
 
-// Grows the context's work buffers to hold a solve of this size.
-int ensure_buffers(emst_ctx* ctx, int32_t nodes, int32_t edges, long long wl_cap, int task_edges);
+// Grows the context's work buffers to hold a solve of this size under the
+// context's configuration (worklists of at least wl_cap records).
+int ensure_buffers(emst_ctx* ctx, int32_t nodes, int32_t edges, long long wl_cap);
 
+// One solve.  With wb == nullptr the context's buffers are grown and used;
+// otherwise wb must hold work_sizes(...) of this solve and nothing is
+// allocated.
 int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
               const int* d_eweight, int use_filter, int32_t threshold, uint8_t* d_inMST, hipStream_t s,
-              long long wl_cap, const VertexThresholds* vt, emst_stats* stats, FixedSchedule* fixed = nullptr);
+              long long wl_cap, const VertexThresholds* vt, emst_stats* stats, FixedSchedule* fixed = nullptr,
+              const WorkBufs* wb = nullptr);
 
 // The reference's kernel sequence with switchable optimizations (the
 // EMST_F_LADDER and EMST_F_NO_* bits of the context's flags): one graph, a

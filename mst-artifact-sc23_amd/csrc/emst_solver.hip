@@ -54,6 +54,13 @@ constexpr int kTbl = 1024;        // LDS min-combining table entries
 constexpr int kPair = 1024;       // LDS parallel-record table entries (k_select after the first round)
 constexpr int kReportWords = ST_COUNT + 3;  // counters, threshold, filter flag, quantile bits
 
+// Dynamic LDS of an initialisation workgroup: per-wave staging and marks,
+// then the workgroup's fill count (16 bytes keep the size a multiple of 16).
+size_t init_lds_bytes(int stage_recs, int task_edges)
+{
+  return (size_t)kInitWaves * stage_recs * sizeof(int4) + (size_t)kInitWaves * task_edges * sizeof(int) + 16;
+}
+
 // ---------------------------------------------------------------------------
 // k_setup: initPM (ECL_MST.cu:147-154) plus the start vertex of every
 // task_edges-edge task: the vertex v whose adjacency range holds the task's
@@ -125,21 +132,30 @@ __global__ __launch_bounds__(256) void k_zero(uint8_t* __restrict__ p, long long
 // lines whatever the degrees, and hub vertices spread over many waves.
 // T comes from the argument, from dthr (decided on the device), or per
 // source vertex from gthr[vgid[v]] when several graphs are solved together.
+// Output: workgroup b appends to its own segment wl[b*seg_cap ...], reserving
+// ranges with an LDS counter, and finally writes its record count to
+// segcnt[b] and adds it to cnt[0].  seg_cap bounds the upper-direction edges
+// of the workgroup's tasks, so a segment never overflows.
 // ---------------------------------------------------------------------------
 template <bool HEAVY, int SUB>
 __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
     const int* __restrict__ nindex, const int* __restrict__ nlist, const int* __restrict__ eweight,
     const int* __restrict__ gsrc, int nodes, int edges, int T, const int* __restrict__ label,
-    const unsigned* __restrict__ vclass, int4* __restrict__ wl, int* __restrict__ cnt, int stage_cap,
-    long long* __restrict__ stats, const int* __restrict__ dthr, const int* __restrict__ vgid,
-    const int* __restrict__ gthr, const int* __restrict__ bigroot_p)
+    const unsigned* __restrict__ vclass, int4* __restrict__ wl, long long seg_cap, int* __restrict__ segcnt,
+    int* __restrict__ cnt, int stage_cap, long long* __restrict__ stats, const int* __restrict__ dthr,
+    const int* __restrict__ vgid, const int* __restrict__ gthr, const int* __restrict__ bigroot_p)
 {
   constexpr int TE = SUB * kWave;
+  // dynamic LDS: per-wave staging, per-wave marks, the workgroup's fill count
   extern __shared__ int4 lds_stage[];
+  int& s_fill = *(reinterpret_cast<int*>(lds_stage + kInitWaves * stage_cap) + kInitWaves * TE);
   if (dthr) {
     if (HEAVY && dthr[1] == 0) return;  // no heavy phase when unfiltered
     T = dthr[0];
   }
+  if (threadIdx.x == 0) s_fill = 0;
+  __syncthreads();
+  int4* const seg = wl + (long long)blockIdx.x * seg_cap;
   const int bigroot = HEAVY ? *bigroot_p : -1;
   const int lane = lane_id();
   const int wid = threadIdx.x / kWave;
@@ -147,7 +163,6 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
   int* marks = reinterpret_cast<int*>(lds_stage + kInitWaves * stage_cap) + wid * TE;
   const long long ngroups = ((long long)edges + TE - 1) / TE;
   const long long nwaves = (long long)gridDim.x * kInitWaves;
-  long long emitted = 0;
   // The stream loads of a wave's next task are issued before the current
   // task is processed (after the current task's boundary window, so waiting
   // for that window does not wait for them).
@@ -248,8 +263,7 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
         const bool up = nl[u] > src[u];
         const int t = (gthr && up) ? gthr[vgid[src[u]]] : T;
         const bool keep = up && w[u] <= t;
-        emitted += keep;
-        stage_push(st, keep, make_int4(src[u], nl[u], w[u], j), wl, cnt);
+        stage_push(st, keep, make_int4(src[u], nl[u], w[u], j), seg, &s_fill);
       }
     } else {
       // The class map resolves most roots without touching parent: a
@@ -280,8 +294,7 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
         const long long p = gbeg + u * kWave + lane;
         const bool keep = up[u] && a[u] != b[u];
         const int wt = keep ? eweight[p] : 0;
-        emitted += keep;
-        stage_push(st, keep, make_int4(a[u], b[u], wt, (int)p), wl, cnt);
+        stage_push(st, keep, make_int4(a[u], b[u], wt, (int)p), seg, &s_fill);
       }
     }
     base = base_n;
@@ -292,8 +305,16 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
       if (!HEAVY) w[u] = w_n[u];
     }
   }
-  stage_flush(st, wl, cnt);
-  if (emitted) atomicAdd((unsigned long long*)&stats[HEAVY ? ST_HEAVY : ST_LIGHT], (unsigned long long)emitted);
+  stage_flush(st, seg, &s_fill);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int f = s_fill;
+    segcnt[blockIdx.x] = f;
+    if (f) {
+      atomicAdd(cnt, f);
+      atomicAdd((unsigned long long*)&stats[HEAVY ? ST_HEAVY : ST_LIGHT], (unsigned long long)f);
+    }
+  }
 }
 
 // Picks the component that the heavy pass' class map marks: the most frequent
@@ -419,6 +440,33 @@ __device__ __forceinline__ bool offer_key(int* tags, u64* keys, u64* minv, int r
 // component's minimum record survives), so rounds and results are those of
 // the reference schedule; only the worklist sizes shrink.  With dedup = 0
 // every crossing record is kept and the sizes are the reference's.
+//
+// Segmented input: a phase's first select and join, and the select after
+// that join, read the initialisation's output, which is nseg segments of
+// seg_cap records each (segment s holds segcnt[s] records at its start).
+// Positions are physical (s*seg_cap + offset); seg_cap is a multiple of the
+// select tile, so a tile never straddles two segments, and the candidate and
+// joined-record bitmasks are indexed by physical position as well.  With
+// segcnt == nullptr the list is dense: cnt_in records from position 0.
+struct ListView {
+  const int* segcnt;
+  long long seg_cap;
+  int nseg;
+};
+
+// Positions below the returned bound hold records, for the tile (or row
+// group) starting at physical position base.
+__device__ __forceinline__ long long list_limit(const ListView& lv, long long base, long long n_dense)
+{
+  if (!lv.segcnt) return n_dense;
+  const long long sidx = base / lv.seg_cap;
+  return sidx * lv.seg_cap + lv.segcnt[sidx];
+}
+
+__device__ __forceinline__ long long list_span(const ListView& lv, long long n_dense)
+{
+  return lv.segcnt ? (long long)lv.nseg * lv.seg_cap : n_dense;
+}
 __device__ __forceinline__ int pair_slot(u64* tags, u64* keys, u64 pair, u64 key)
 {
   const unsigned h = ((unsigned)(pair ^ (pair >> 29)) * 0x9E3779B1u) >> (32 - 10);
@@ -442,11 +490,11 @@ template <bool FIRST, int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void k_select(
     const int4* __restrict__ wl_in, const int* __restrict__ cnt_in, int4* __restrict__ wl_out,
     int* __restrict__ cnt_out, const int* __restrict__ parent, u64* __restrict__ minv, int guard, int combine,
-    int dedup, long long* __restrict__ stats, int rounds_slot, u64* __restrict__ cand, const u64* __restrict__ dead)
+    int dedup, long long* __restrict__ stats, int rounds_slot, u64* __restrict__ cand, const u64* __restrict__ dead,
+    ListView lv)
 {
   constexpr int BT = WAVES * kWave;
   constexpr int TILE = BT * kSelItems;
-  const long long cand_words = ((long long)*cnt_in + kWave - 1) / kWave;
   __shared__ int s_tags[kTbl];
   __shared__ u64 s_keys[kTbl];
   __shared__ int s_wtot[WAVES];
@@ -456,6 +504,8 @@ __global__ __launch_bounds__(WAVES * kWave) void k_select(
   __shared__ u64 s_pkey[FIRST ? 1 : kPair];
   const int n_in = *cnt_in;
   if (n_in == 0) return;
+  const long long span = list_span(lv, n_in);
+  const long long cand_words = (span + kWave - 1) / kWave;
   const int tid = threadIdx.x;
   if (blockIdx.x == 0 && tid == 0) {
     stats[rounds_slot] += 1;
@@ -473,16 +523,18 @@ __global__ __launch_bounds__(WAVES * kWave) void k_select(
   __syncthreads();
   const int lane = lane_id();
   const int wid = tid / kWave;
-  const long long ntiles = ((long long)n_in + TILE - 1) / TILE;
+  const long long ntiles = (span + TILE - 1) / TILE;
   for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const long long base = t * TILE;
+    const long long lim = list_limit(lv, base, n_in);
+    if (base >= lim) continue;  // an empty part of a segment (uniform per tile)
     int4 r[kSelItems];
     bool live[kSelItems];
 #pragma unroll
     for (int k = 0; k < kSelItems; k++) {
       const long long i = base + k * BT + tid;
       // records joined by the first round are known to lie inside one tree
-      live[k] = i < n_in && !(dead && ((dead[(base + k * BT) / kWave + wid] >> lane) & 1ull));
+      live[k] = i < lim && !(dead && ((dead[(base + k * BT) / kWave + wid] >> lane) & 1ull));
       r[k] = live[k] ? wl_in[i] : make_int4(0, 0, 0, 0);
     }
     bool keep[kSelItems];
@@ -650,10 +702,12 @@ template <int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void k_join(
     int4* __restrict__ wl, int* __restrict__ cnt, int4* __restrict__ wl_other, int* __restrict__ cnt_other,
     int* __restrict__ parent, u64* __restrict__ minv, uint8_t* __restrict__ inMST, int tail_max,
-    int* __restrict__ hflag, long long* __restrict__ stats, int rounds_slot, const u64* cand, u64* joined_mask)
+    int* __restrict__ hflag, long long* __restrict__ stats, int rounds_slot, const u64* cand, u64* joined_mask,
+    ListView lv)
 {
+  constexpr int BT = WAVES * kWave;
   const int n = *cnt;
-  if (n > 0 && n <= tail_max) {
+  if (!lv.segcnt && n > 0 && n <= tail_max) {
     if (blockIdx.x != 0) return;
     if (threadIdx.x == 0) {
       stats[ST_OUT_SUM] += n;
@@ -671,13 +725,20 @@ __global__ __launch_bounds__(WAVES * kWave) void k_join(
     stats[ST_OUT_SUM] += n;
     __hip_atomic_store(hflag, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  const long long cand_words = ((long long)n + kWave - 1) / kWave;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+  if (n == 0) return;  // (segment counts are only valid for a non-empty list)
+  const long long span = list_span(lv, n);
+  const long long cand_words = (span + kWave - 1) / kWave;
+  const long long ntiles = (span + BT - 1) / BT;
+  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const long long base = t * BT;
+    const long long lim = list_limit(lv, base, n);
+    if (base >= lim) continue;
+    const long long i = base + threadIdx.x;
+    const bool live = i < lim;
     // after the offer-only first round, only a candidate endpoint can have
     // this record as its minimum; later rounds check both endpoints
-    bool cx = true, cy = true;
-    if (cand) {
+    bool cx = live, cy = live;
+    if (cand && live) {
       cx = (cand[i / kWave] >> (i % kWave)) & 1ull;
       cy = (cand[cand_words + i / kWave] >> (i % kWave)) & 1ull;
     }
@@ -699,10 +760,9 @@ __global__ __launch_bounds__(WAVES * kWave) void k_join(
     // A joined record lies inside one tree from now on: its bit tells the
     // next select to drop it without finds.  In the first round the mask
     // overwrites the candidate word this wavefront alone has just read.
-    // Lane 0 holds the row's first record, so it is active whenever any
-    // lane is.
+    // Rows that start at or past the limit hold no record and write no word.
     const u64 jm = __ballot(joined);
-    if (lane_id() == 0) joined_mask[i / kWave] = jm;
+    if (lane_id() == 0 && i < lim) joined_mask[i / kWave] = jm;
   }
 }
 
@@ -720,34 +780,14 @@ __global__ __launch_bounds__(64) void k_report(const long long* __restrict__ sta
   if (t < kReportWords) __hip_atomic_store(&out[t], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// Host-sync ablation: a plain kernel2 without the fused tail.
-__global__ __launch_bounds__(256) void k_join_plain(const int4* __restrict__ wl, int n, int* __restrict__ parent,
-                                                    u64* __restrict__ minv, uint8_t* __restrict__ inMST)
-{
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    const int4 r = wl[i];
-    const int x = r.x, y = r.y;
-    const u64 key = make_key(r.z, r.w);
-    const u64 kx = minv[x];
-    const u64 ky = minv[y];
-    const bool wx = key == kx, wy = key == ky;
-    if (wx) minv[x] = kNoKey;
-    if (wy) minv[y] = kNoKey;
-    if (wx || wy) {
-      hook(x, y, parent);
-      inMST[r.w] = 1;
-    }
-  }
-}
-
 // ---- runtime selection of template instances -------------------------------
 typedef void (*InitFn)(const int*, const int*, const int*, const int*, int, int, int, const int*,
-                       const unsigned*, int4*, int*, int, long long*, const int*, const int*, const int*,
-                       const int*);
+                       const unsigned*, int4*, long long, int*, int*, int, long long*, const int*, const int*,
+                       const int*, const int*);
 typedef void (*SelFn)(const int4*, const int*, int4*, int*, const int*, u64*, int, int, int, long long*, int, u64*,
-                      const u64*);
-typedef void (*JoinFn)(int4*, int*, int4*, int*, int*, u64*, uint8_t*, int, int*, long long*, int, const u64*, u64*);
+                      const u64*, ListView);
+typedef void (*JoinFn)(int4*, int*, int4*, int*, int*, u64*, uint8_t*, int, int*, long long*, int, const u64*, u64*,
+                       ListView);
 
 InitFn init_fn(bool heavy, int sub)
 {
@@ -827,106 +867,147 @@ emst_config default_config()
 
 namespace emst {
 
-// The work buffers are managed in one function so their capacities stay in step.
-int ensure_buffers(emst_ctx* ctx, int32_t nodes, int32_t edges, long long wl_cap, int task_edges)
+// Element counts of one solve's work buffers (see WorkSizes).  The
+// initialisation grid is at most one workgroup per four tasks; workgroup b
+// runs tasks b*kInitWaves + wave + k*kInitWaves*grid, so its segment needs
+// room for kInitWaves*ceil(tasks / (kInitWaves*grid)) tasks of tile_edges
+// positions, rounded up to a whole select tile.
+WorkSizes work_sizes(int cus, const emst_config& cfg, int32_t nodes, int32_t edges, long long wl_cap)
 {
-  if ((size_t)nodes > ctx->cap_nodes || !ctx->d_parent) {
+  WorkSizes z;
+  z.nodes = std::max<long long>(nodes, 1);
+  z.vclass_words = (z.nodes + 63) / 16;
+  z.gsrc = ((long long)edges + 255) / 256 + 1;  // room for the smallest task size
+  const long long te = cfg.tile_edges;
+  const long long tasks = ((long long)edges + te - 1) / te;
+  const long long grid_max = std::max(1, cus * std::max(1, cfg.init_grid_per_cu));
+  const long long grid = std::max<long long>(1, std::min(grid_max, (tasks + kInitWaves - 1) / kInitWaves));
+  const long long per = kInitWaves * ((tasks + kInitWaves * grid - 1) / (kInitWaves * grid));
+  const long long tile = (long long)cfg.block * kSelItems;
+  z.init_grid = (int)grid;
+  z.seg_cap = std::max<long long>((per * te + tile - 1) / tile * tile, tile);
+  z.segs = grid;
+  z.wl[1] = std::max<long long>(wl_cap, 1);
+  z.wl[0] = std::max<long long>(z.wl[1], grid * z.seg_cap);
+  z.cand_words = (z.wl[0] + kWave - 1) / kWave;
+  return z;
+}
+
+// The context's work buffers are managed in one function so their
+// capacities stay in step; any reallocation invalidates captured schedules.
+int ensure_buffers(emst_ctx* ctx, int32_t nodes, int32_t edges, long long wl_cap)
+{
+  const WorkSizes z = work_sizes(ctx->cus, ctx->cfg, nodes, edges, wl_cap);
+  auto grow = [&](void** p, size_t* cap, size_t want, size_t elem, const char* what) -> int {
+    if (*p && *cap >= want) return EMST_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    if (hipMalloc(p, want * elem) != hipSuccess) {
+      char buf[160];
+      std::snprintf(buf, sizeof(buf), "hipMalloc of %zu bytes (%s) failed", want * elem, what);
+      return ctx_fail(ctx, EMST_E_NOMEM, buf);
+    }
+    *cap = want;
+    ctx->buf_gen++;
+    return EMST_OK;
+  };
+  if ((size_t)z.nodes > ctx->cap_nodes || !ctx->d_parent) {
     for (void** p : {(void**)&ctx->d_parent, (void**)&ctx->d_minv, (void**)&ctx->d_vclass}) {
       if (*p) (void)hipFree(*p);
       *p = nullptr;
     }
     ctx->cap_nodes = 0;
-    const size_t n = (size_t)std::max(nodes, 1);
-    if (hipMalloc((void**)&ctx->d_parent, n * 4) != hipSuccess || hipMalloc((void**)&ctx->d_minv, n * 8) != hipSuccess ||
-        hipMalloc((void**)&ctx->d_vclass, (n + 63) / 16 * 4) != hipSuccess)
+    if (hipMalloc((void**)&ctx->d_parent, z.nodes * 4) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_minv, z.nodes * 8) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_vclass, z.vclass_words * 4) != hipSuccess)
       return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of parent/minv failed");
-    ctx->cap_nodes = n;
+    ctx->cap_nodes = z.nodes;
     ctx->buf_gen++;
   }
-  const size_t groups = ((size_t)edges + task_edges - 1) / task_edges;
-  if (groups > ctx->cap_gsrc || !ctx->d_gsrc) {
-    if (ctx->d_gsrc) (void)hipFree(ctx->d_gsrc);
-    ctx->d_gsrc = nullptr;
-    ctx->cap_gsrc = 0;
-    const size_t want = std::max<size_t>(groups, 1) * 4;  // room for smaller task sizes later
-    if (hipMalloc((void**)&ctx->d_gsrc, want * 4) != hipSuccess)
-      return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of task start vertices failed");
-    ctx->cap_gsrc = want;
-    ctx->buf_gen++;
-  }
-  if ((size_t)wl_cap > ctx->cap_wl || !ctx->d_wl[0]) {
-    for (int i = 0; i < 2; i++) {
-      if (ctx->d_wl[i]) (void)hipFree(ctx->d_wl[i]);
-      ctx->d_wl[i] = nullptr;
-    }
-    ctx->cap_wl = 0;
-    const size_t want = std::max<size_t>((size_t)wl_cap, 1);
-    for (int i = 0; i < 2; i++)
-      if (hipMalloc((void**)&ctx->d_wl[i], want * sizeof(int4)) != hipSuccess) {
-        char buf[128];
-        std::snprintf(buf, sizeof(buf), "hipMalloc of a %zu-record worklist failed", want);
-        return ctx_fail(ctx, EMST_E_NOMEM, buf);
-      }
-    if (ctx->d_cand) (void)hipFree(ctx->d_cand);
-    ctx->d_cand = nullptr;
-    // two candidate bitmasks of the first round, one bit per record
-    if (hipMalloc((void**)&ctx->d_cand, 2 * ((want + kWave - 1) / kWave) * sizeof(u64)) != hipSuccess)
-      return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of the candidate bitmasks failed");
-    ctx->cap_wl = want;
-    ctx->buf_gen++;
-  }
+  int rc;
+  if ((rc = grow((void**)&ctx->d_gsrc, &ctx->cap_gsrc, z.gsrc, 4, "task start vertices"))) return rc;
+  if ((rc = grow((void**)&ctx->d_wl[0], &ctx->cap_wl[0], z.wl[0], sizeof(int4), "worklist 0"))) return rc;
+  if ((rc = grow((void**)&ctx->d_wl[1], &ctx->cap_wl[1], z.wl[1], sizeof(int4), "worklist 1"))) return rc;
+  if ((rc = grow((void**)&ctx->d_cand, &ctx->cap_cand, 2 * z.cand_words, 8, "candidate bitmasks"))) return rc;
+  if ((rc = grow((void**)&ctx->d_segcnt, &ctx->cap_seg, z.segs, 4, "segment counts"))) return rc;
   return EMST_OK;
 }
 
 }  // namespace emst
 
-namespace {
-}  // namespace
-
 namespace emst {
 
 int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
               const int* d_eweight, int use_filter, int32_t threshold, uint8_t* d_inMST, hipStream_t s,
-              long long wl_cap, const VertexThresholds* vt, emst_stats* stats, FixedSchedule* fixed)
+              long long wl_cap, const VertexThresholds* vt, emst_stats* stats, FixedSchedule* fixed,
+              const WorkBufs* wbp)
 {
   if (nodes < 1 || edges < 0) return ctx_fail(ctx, EMST_E_ARG, "node or edge count too low");
   if (use_filter < 0 || use_filter > EMST_FILTER_LEARNED) return ctx_fail(ctx, EMST_E_ARG, "bad filter mode");
   const emst_config& cfg = ctx->cfg;
   if (cfg.flags & kLadderFlags) {
-    if (use_filter == EMST_FILTER_LEARNED || vt || fixed)
+    if (use_filter == EMST_FILTER_LEARNED || vt || fixed || wbp)
       return ctx_fail(ctx, EMST_E_ARG, "the ablation ladder takes one graph and a given threshold");
-    return run_ladder(ctx, nodes, edges, d_nindex, d_nlist, d_eweight, use_filter, threshold, d_inMST, s, wl_cap,
-                      stats);
+    const int rc = run_ladder(ctx, nodes, edges, d_nindex, d_nlist, d_eweight, use_filter, threshold, d_inMST, s,
+                              wl_cap, stats);
+    if (rc == EMST_OK) {
+      ctx->last_nodes = nodes;
+      ctx->last_parent = ctx->d_parent;
+    }
+    return rc;
   }
   const int task_edges = cfg.tile_edges;
   const int sub = task_edges / kWave;
   const int waves = cfg.block / kWave;
-  int rc = ensure_buffers(ctx, nodes, edges, wl_cap, task_edges);
-  if (rc) return rc;
+  WorkBufs wb;
+  if (wbp) {
+    wb = *wbp;
+  } else {
+    int rc = ensure_buffers(ctx, nodes, edges, wl_cap);
+    if (rc) return rc;
+    wb.parent = ctx->d_parent;
+    wb.minv = ctx->d_minv;
+    wb.vclass = ctx->d_vclass;
+    wb.gsrc = ctx->d_gsrc;
+    wb.wl[0] = ctx->d_wl[0];
+    wb.wl[1] = ctx->d_wl[1];
+    wb.cand = ctx->d_cand;
+    wb.cnt = ctx->d_cnt;
+    wb.stats = ctx->d_stats;
+    wb.segcnt = ctx->d_segcnt;
+  }
+  const WorkSizes z = work_sizes(ctx->cus, cfg, nodes, edges, wl_cap);
   const bool learned = use_filter == EMST_FILTER_LEARNED && !vt;
   const int T = use_filter == EMST_FILTER_PLAIN ? threshold : INT_MAX;
   const int* dthr = learned ? ctx->d_thr : nullptr;
+  // Host-sync ablation (the reference reads the worklist size back after
+  // every kernel1, ECL_MST.cu:318): the host waits for every round's
+  // posted size before it enqueues the next round, and no round runs in the
+  // one-workgroup tail.
   const bool host_sync = (cfg.flags & EMST_F_HOST_SYNC) != 0;
   if (host_sync && (learned || vt || fixed))
     return ctx_fail(ctx, EMST_E_ARG, "host-sync mode takes one graph and a given threshold");
   const int guard = (cfg.flags & EMST_F_NO_GUARD) ? 0 : 1;
   const int combine = (cfg.flags & EMST_F_NO_COMBINE) ? 0 : 1;
   const int dedup = (cfg.flags & EMST_F_NO_PAIR_DEDUP) ? 0 : 1;
-  const int tail_max = cfg.tail_max < 0 ? 0 : cfg.tail_max;
+  const int tail_max = host_sync || cfg.tail_max < 0 ? 0 : cfg.tail_max;
   const int round_grid = std::max(1, ctx->cus * std::max(1, cfg.grid_per_cu));
-  const int init_grid = std::max(1, ctx->cus * std::max(1, cfg.init_grid_per_cu));
   const int stage_cap = std::max(64, cfg.stage_recs);
-  const size_t init_lds = (size_t)kInitWaves * stage_cap * sizeof(int4) + (size_t)kInitWaves * task_edges * sizeof(int);
+  const size_t init_lds = init_lds_bytes(stage_cap, task_edges);
   if (init_lds > ctx->lds_per_block) return ctx_fail(ctx, EMST_E_ARG, "stage_recs exceeds the LDS of a workgroup");
   const int nphases = (vt ? vt->any_filter : use_filter != EMST_FILTER_OFF) ? 2 : 1;
   const int* vgid = vt ? vt->vgid : nullptr;
   const int* gthr = vt ? vt->gthr : nullptr;
-  int* d_cnt = ctx->d_cnt;
-  long long* d_stats = ctx->d_stats;
-  int4** wl = ctx->d_wl;
+  int* d_cnt = wb.cnt;
+  long long* d_stats = wb.stats;
+  int4** wl = wb.wl;
   ctx->last_nodes = nodes;
+  ctx->last_parent = wb.parent;
   const SelFn sel_first = sel_fn(true, waves), sel = sel_fn(false, waves);
   const JoinFn join = join_fn(waves);
+  const ListView dense{nullptr, 0, 0};
+  const ListView segs{wb.segcnt, z.seg_cap, z.init_grid};
 
   if (!fixed) HIPCHK(hipEventRecord(ctx->ev[0], s));
   if (learned) {
@@ -939,8 +1020,8 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
   }
   {
     const int blocks = (int)(((long long)nodes + 255) / 256);
-    hipLaunchKernelGGL(k_setup, dim3(blocks), dim3(256), 0, s, nodes, d_nindex, ctx->d_parent, ctx->d_minv,
-                       ctx->d_gsrc, task_edges, d_cnt, d_stats);
+    hipLaunchKernelGGL(k_setup, dim3(blocks), dim3(256), 0, s, nodes, d_nindex, wb.parent, wb.minv, wb.gsrc,
+                       task_edges, d_cnt, d_stats);
     const long long z16 = ((long long)edges + 15) / 16;
     const int zb = (int)std::min<long long>(std::max<long long>((z16 + 255) / 256, 1), (long long)ctx->cus * 16);
     hipLaunchKernelGGL(k_zero, dim3(zb), dim3(256), 0, s, d_inMST, (long long)edges);
@@ -965,70 +1046,46 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
     }
     if (phase == 1) {
       const int blocks = (int)(((long long)nodes + 255) / 256);
-      hipLaunchKernelGGL(k_pick_big, dim3(1), dim3(kWave), 0, s, nodes, ctx->d_parent, &d_cnt[2], dthr);
-      hipLaunchKernelGGL(k_flatten, dim3(blocks), dim3(256), 0, s, nodes, ctx->d_parent, d_cnt, &d_cnt[2],
-                         ctx->d_vclass, dthr);
+      hipLaunchKernelGGL(k_pick_big, dim3(1), dim3(kWave), 0, s, nodes, wb.parent, &d_cnt[2], dthr);
+      hipLaunchKernelGGL(k_flatten, dim3(blocks), dim3(256), 0, s, nodes, wb.parent, d_cnt, &d_cnt[2], wb.vclass,
+                         dthr);
     }
     if (edges > 0) {
       if (!fixed) HIPCHK(hipEventRecord(ctx->ev[2 + 2 * phase], s));
       const InitFn f = init_fn(phase == 1, sub);
-      hipLaunchKernelGGL(f, dim3(init_grid), dim3(kInitWaves * kWave), init_lds, s, d_nindex, d_nlist, d_eweight,
-                         ctx->d_gsrc, nodes, edges, T, ctx->d_parent, ctx->d_vclass, wl[0], &d_cnt[0], stage_cap,
-                         d_stats, dthr, phase == 0 ? vgid : nullptr, phase == 0 ? gthr : nullptr, &d_cnt[2]);
+      hipLaunchKernelGGL(f, dim3(z.init_grid), dim3(kInitWaves * kWave), init_lds, s, d_nindex, d_nlist, d_eweight,
+                         wb.gsrc, nodes, edges, T, wb.parent, wb.vclass, wl[0], z.seg_cap, wb.segcnt, &d_cnt[0],
+                         stage_cap, d_stats, dthr, phase == 0 ? vgid : nullptr, phase == 0 ? gthr : nullptr,
+                         &d_cnt[2]);
       if (!fixed) HIPCHK(hipEventRecord(ctx->ev[3 + 2 * phase], s));
       HIPCHK(hipGetLastError());
       init_launches++;
     }
     const int rounds_slot = phase == 0 ? ST_ROUNDS0 : ST_ROUNDS1;
-    if (host_sync) {
-      // Reference schedule: read the worklist size back after every kernel1.
-      int cur = 0;
-      int size = 0;
-      HIPCHK(hipMemcpyAsync(&size, &d_cnt[cur], 4, hipMemcpyDeviceToHost, s));
-      HIPCHK(hipStreamSynchronize(s));
-      const int tile = waves * kWave * kSelItems;
-      if (size > 0) {
-        const int blocks = (int)std::min<long long>(((long long)size + tile - 1) / tile, round_grid);
-        hipLaunchKernelGGL(sel_first, dim3(blocks), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[cur ^ 1],
-                           &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, combine, dedup, d_stats, rounds_slot,
-                           ctx->d_cand, (const u64*)nullptr);
-      }
-      while (size > 0) {
-        hipLaunchKernelGGL(k_join_plain, dim3((size + 255) / 256), dim3(256), 0, s, wl[cur], size, ctx->d_parent,
-                           ctx->d_minv, d_inMST);
-        HIPCHK(hipMemsetAsync(&d_cnt[cur ^ 1], 0, 4, s));
-        const int blocks = (int)std::min<long long>(((long long)size + tile - 1) / tile, round_grid);
-        hipLaunchKernelGGL(sel, dim3(blocks), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[cur ^ 1],
-                           &d_cnt[cur ^ 1], ctx->d_parent, ctx->d_minv, guard, combine, dedup, d_stats, rounds_slot,
-                           (u64*)nullptr, (const u64*)nullptr);
-        cur ^= 1;
-        HIPCHK(hipMemcpyAsync(&size, &d_cnt[cur], 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-      }
-      HIPCHK(hipMemsetAsync(d_cnt, 0, 8, s));
-      continue;
-    }
-    // Device-driven rounds with a bounded look-ahead.
-    const int lag = 2;
+    // Device-driven rounds with a bounded look-ahead (none in host-sync
+    // mode).  The phase's first select and join and the select after them
+    // read the initialisation's segments; later lists are dense.
+    const int lag = host_sync ? 0 : 2;
     const int first_slot = slot;
     for (int r = 0;; r++) {
       if (slot >= kHflagSlots) return ctx_fail(ctx, EMST_E_LIMIT, "round limit exceeded");
       // list `cur` holds the survivors of the last select; the phase's
       // first select is the offer-only pass over the initial list
       const int cur = r & 1, nxt = cur ^ 1;
+      const ListView lv = r == 0 ? segs : dense;
       ctx->h_flag[slot] = -1;
       if (r == 0)
         hipLaunchKernelGGL(sel_first, dim3(round_grid), dim3(waves * kWave), 0, s, wl[0], &d_cnt[0], wl[1], &d_cnt[1],
-                           ctx->d_parent, ctx->d_minv, guard, combine, dedup, d_stats, rounds_slot, ctx->d_cand,
-                           (const u64*)nullptr);
+                           wb.parent, wb.minv, guard, combine, dedup, d_stats, rounds_slot, wb.cand,
+                           (const u64*)nullptr, lv);
       // the first join reads the candidate bitmasks; every join marks what it joined
       hipLaunchKernelGGL(join, dim3(round_grid), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt], &d_cnt[nxt],
-                         ctx->d_parent, ctx->d_minv, d_inMST, tail_max, ctx->d_flag + slot, d_stats, rounds_slot,
-                         (const u64*)(r == 0 ? ctx->d_cand : nullptr), ctx->d_cand);
+                         wb.parent, wb.minv, d_inMST, tail_max, ctx->d_flag + slot, d_stats, rounds_slot,
+                         (const u64*)(r == 0 ? wb.cand : nullptr), wb.cand, lv);
       // each select drops the records the join before it joined
       hipLaunchKernelGGL(sel, dim3(round_grid), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt], &d_cnt[nxt],
-                         ctx->d_parent, ctx->d_minv, guard, combine, dedup, d_stats, rounds_slot, (u64*)nullptr,
-                         (const u64*)ctx->d_cand);
+                         wb.parent, wb.minv, guard, combine, dedup, d_stats, rounds_slot, (u64*)nullptr,
+                         (const u64*)wb.cand, lv);
       HIPCHK(hipGetLastError());
       slot++;
       if (fixed) {
@@ -1155,7 +1212,7 @@ void emst_destroy(emst_ctx* ctx)
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  void* ptrs[] = {ctx->d_cand,   ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, ctx->d_inMST, ctx->d_parent,
+  void* ptrs[] = {ctx->d_cand,   ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, ctx->d_inMST, ctx->d_parent, ctx->d_segcnt,
                   ctx->d_minv,   ctx->d_wl[0], ctx->d_wl[1],   ctx->d_gsrc,  ctx->d_cnt,
                   ctx->d_stats,  ctx->d_vclass, ctx->d_mlp16,  ctx->d_thr,   ctx->d_q,
                   ctx->d_feat,   ctx->d_samples, ctx->d_lad[0], ctx->d_lad[1], ctx->d_slist,
@@ -1191,8 +1248,7 @@ int emst_set_config(emst_ctx* ctx, const emst_config* cfg)
   if (c.flags & ~kAllFlags) return ctx_fail(ctx, EMST_E_ARG, "unknown flag bits");
   if (c.grid_per_cu < 1 || c.grid_per_cu > 64 || c.init_grid_per_cu < 1 || c.init_grid_per_cu > 64)
     return ctx_fail(ctx, EMST_E_ARG, "grid_per_cu out of range");
-  if (c.stage_recs < 64 ||
-      (size_t)kInitWaves * (c.stage_recs * sizeof(int4) + c.tile_edges * sizeof(int)) > ctx->lds_per_block)
+  if (c.stage_recs < 64 || init_lds_bytes(c.stage_recs, c.tile_edges) > ctx->lds_per_block)
     return ctx_fail(ctx, EMST_E_ARG, "stage_recs out of range");
   if (c.tail_max > (1 << 20)) return ctx_fail(ctx, EMST_E_ARG, "tail_max out of range");
   if (c.tile_edges != 256 && c.tile_edges != 512 && c.tile_edges != 1024)
@@ -1257,7 +1313,7 @@ int emst_load(emst_ctx* ctx, const emst_graph* g)
   ctx->m = g->edges;
   ctx->m_up = up;
   ctx->loaded = true;
-  return ensure_buffers(ctx, g->nodes, g->edges, up, ctx->cfg.tile_edges);
+  return ensure_buffers(ctx, g->nodes, g->edges, up);
 }
 
 int emst_solve(emst_ctx* ctx, int use_filter, int32_t threshold, emst_stats* stats)
@@ -1282,9 +1338,9 @@ int emst_fetch(emst_ctx* ctx, uint8_t* inMST)
 int emst_fetch_parent(emst_ctx* ctx, int32_t* parent)
 {
   if (!ctx || !parent) return EMST_E_ARG;
-  if (!ctx->d_parent || ctx->last_nodes < 1) return ctx_fail(ctx, EMST_E_STATE, "no solve has run");
+  if (!ctx->last_parent || ctx->last_nodes < 1) return ctx_fail(ctx, EMST_E_STATE, "no solve has run");
   HIPCHK(hipSetDevice(ctx->device));
-  HIPCHK(hipMemcpyAsync(parent, ctx->d_parent, (size_t)ctx->last_nodes * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(parent, ctx->last_parent, (size_t)ctx->last_nodes * 4, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return EMST_OK;
 }
@@ -1320,7 +1376,10 @@ int emst_set_filter_model(emst_ctx* ctx, const float* params)
     h[i] = emst_bf16_bits(params[i]);
   }
   HIPCHK(hipSetDevice(ctx->device));
-  HIPCHK(hipMemcpy(ctx->d_mlp16, h.data(), kParams * 2, hipMemcpyHostToDevice));
+  // the solve kernels read the parameters on the context's non-blocking
+  // stream, so the copy is ordered on that stream and h outlives it
+  HIPCHK(hipMemcpyAsync(ctx->d_mlp16, h.data(), kParams * 2, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->have_model = 1;
   return EMST_OK;
 }

@@ -502,7 +502,7 @@ int run_ladder(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex,
   // the size/flag words
   const long long recs = md.one_dir ? std::min<long long>(std::max<long long>(wl_cap, 0), edges) : edges;
   const size_t rec_bytes = md.tuples ? sizeof(int4) : sizeof(int);
-  int rc = ensure_buffers(ctx, nodes, edges, 1, ctx->cfg.tile_edges);
+  int rc = ensure_buffers(ctx, nodes, edges, 1);
   if (rc) return rc;
   if (md.edge_based && !(md.tuples == false && md.data_driven == false))
     for (int i = 0; i < 2; i++)

@@ -95,7 +95,7 @@ def test_device_features_match_oracle():
         assert np.array_equal(samp, w)
         assert np.allclose(feat, f, rtol=1e-6, atol=1e-7)
         # no model loaded: the device falls back to the plain rule
-        assert (uf, T) == emst.plain_threshold(g) and q < 0
+        assert (uf, T) == O.plain_threshold(g.nodes, g.eweight) and q < 0
     s.close()
 
 
