@@ -201,6 +201,27 @@ int emst_solve_device(emst_ctx* ctx, int32_t nodes, int32_t edges, const int32_t
                       const int32_t* d_nlist, const int32_t* d_eweight, int use_filter,
                       int32_t threshold, uint8_t* d_inMST, void* stream, emst_stats* stats);
 
+/* Allocation-free form of emst_solve_device.  No counterpart in the
+ * reference (its gpuMST allocates every buffer per call, ECL_MST.cu:268-297):
+ * it is for callers that cannot allocate device memory -- code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.  Same
+ * arguments and results as emst_solve_device, but every work buffer
+ * (parents, minimum keys, worklists, bitmasks, counters) is carved from one
+ * static device array of EMST_SCRATCH_BYTES per GPU, held until the process
+ * ends, so the call makes no hipMalloc, hipFree or other allocating call.
+ * A solve needs about 12.3 bytes per vertex plus 32.3 bytes per directed
+ * edge with the default configuration (a 512x512 grid: 37 MB); a larger
+ * graph is refused with EMST_E_LIMIT, whose message gives the bytes needed.
+ * Every call on one GPU shares that GPU's array and leaves its contents
+ * behind (each solve clears what must start at zero): calls that overlap on
+ * one GPU, from any context or stream, race, and the caller must serialise
+ * them.  The ablation ladder flags are refused (EMST_E_ARG): the ladder
+ * allocates its own lists. */
+#define EMST_SCRATCH_BYTES ((size_t)64 << 20)
+int emst_solve_device_noalloc(emst_ctx* ctx, int32_t nodes, int32_t edges, const int32_t* d_nindex,
+                              const int32_t* d_nlist, const int32_t* d_eweight, int use_filter,
+                              int32_t threshold, uint8_t* d_inMST, void* stream, emst_stats* stats);
+
 /* Device parent array of the last solve (ints, nodes entries): after a
  * solve every vertex's root is the smallest vertex id of its tree
  * (join() at ECL_MST.cu:137-144 hooks the larger root under the smaller). */

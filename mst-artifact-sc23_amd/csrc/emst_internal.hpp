@@ -135,6 +135,11 @@ int run_ladder(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex,
                const int* d_eweight, int use_filter, int32_t threshold, uint8_t* d_inMST, hipStream_t s,
                long long wl_cap, emst_stats* stats);
 
+// Takes the device address of this device's static scratch array for the
+// allocation-free solve (noalloc.hip); called once per device, with the
+// device current, when a context is created.
+int noalloc_setup(int device);
+
 // Flags that select the ladder schedule.
 constexpr int kLadderFlags = EMST_F_LADDER | EMST_F_NO_WARP | EMST_F_NO_IMPLICIT_PC | EMST_F_NO_ONE_DIRECTION |
                              EMST_F_NO_TUPLES | EMST_F_NO_DATA_DRIVEN | EMST_F_NO_EDGE_BASED;

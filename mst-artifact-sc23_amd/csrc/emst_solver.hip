@@ -1179,6 +1179,7 @@ int emst_create(int device, emst_ctx** out)
   ctx->cus = prop.multiProcessorCount;
   ctx->lds_per_block = prop.sharedMemPerBlock;
   bool ok = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) == hipSuccess;
+  ok = ok && noalloc_setup(device) == EMST_OK;
   ok = ok && hipMalloc((void**)&ctx->d_cnt, 16) == hipSuccess;
   ok = ok && hipMalloc((void**)&ctx->d_stats, ST_COUNT * sizeof(long long)) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&ctx->h_flag, kHflagSlots * sizeof(int), hipHostMallocMapped) == hipSuccess;

@@ -127,6 +127,9 @@ _fetch = _sig("emst_fetch", C.c_int, C.c_void_p, _pu8)
 _fetch_parent = _sig("emst_fetch_parent", C.c_int, C.c_void_p, _p32)
 _solve_device = _sig("emst_solve_device", C.c_int, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                      C.c_void_p, C.c_int, C.c_int32, C.c_void_p, C.c_void_p, C.POINTER(Stats))
+_solve_device_noalloc = _sig("emst_solve_device_noalloc", C.c_int, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
+                             C.c_void_p, C.c_void_p, C.c_int, C.c_int32, C.c_void_p, C.c_void_p, C.POINTER(Stats))
+SCRATCH_BYTES = 64 << 20  # EMST_SCRATCH_BYTES
 _read_egr = _sig("emst_read_egr", C.c_int, C.c_char_p, C.POINTER(_GraphBuf))
 _write_egr = _sig("emst_write_egr", C.c_int, C.c_char_p, C.POINTER(_Graph))
 _writers = {fmt: _sig(f"emst_write_{fmt}", C.c_int, C.c_char_p, C.POINTER(_Graph))
@@ -413,6 +416,16 @@ class Solver:
         self._err(_solve_device(self._h, nodes, edges, C.c_void_p(d_nindex), C.c_void_p(d_nlist),
                                 C.c_void_p(d_eweight), int(use_filter), int(threshold),
                                 C.c_void_p(d_inmst), C.c_void_p(stream) if stream else None, C.byref(st)))
+        return st
+
+    def solve_device_noalloc(self, nodes, edges, d_nindex, d_nlist, d_eweight, use_filter, threshold, d_inmst,
+                             stream=None):
+        """emst_solve_device_noalloc: the same solve with every work buffer
+        in the library's static scratch array (no device allocation)."""
+        st = Stats()
+        self._err(_solve_device_noalloc(self._h, nodes, edges, C.c_void_p(d_nindex), C.c_void_p(d_nlist),
+                                        C.c_void_p(d_eweight), int(use_filter), int(threshold),
+                                        C.c_void_p(d_inmst), C.c_void_p(stream) if stream else None, C.byref(st)))
         return st
 
 
