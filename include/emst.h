@@ -86,7 +86,7 @@ typedef struct {
   int32_t threshold_used;    /* light-edge threshold the solve used (INT32_MAX = none)  */
   int32_t filter_used;       /* 1 when the two-phase light/heavy path ran              */
   float learned_q;           /* quantile predicted by the network, -1 if not used      */
-  int32_t reserved;
+  int32_t sparse_heavy;      /* 1 when the sparse heavy initialisation ran             */
 } emst_stats;
 
 /* Tunables.  Zero in a field selects the built-in default. */
@@ -113,6 +113,13 @@ typedef struct {
  * worklist sizes (round_in_sum, round_out_sum) are the reference schedule's
  * (ECL_MST.cu:210-226 keeps every crossing record). */
 #define EMST_F_NO_PAIR_DEDUP 1024
+/* Always use the full heavy initialisation.  By default, when the loaded
+ * graph's lists are sorted and symmetric (checked by emst_load, which then
+ * builds the position of every edge's upper-direction copy), the heavy
+ * initialisation examines only the adjacency of vertices outside the
+ * largest light-phase component and skips the rest of the CSR.  Results,
+ * records and round counts are the same either way (ablation). */
+#define EMST_F_NO_SPARSE_HEAVY 2048
 
 /* The de-optimization ladder (reference: Deoptimizations/ECL_MST_No*.cu,
  * removing one optimization per file).  EMST_F_LADDER runs the reference's

@@ -38,6 +38,14 @@ struct emst_ctx {
   size_t cap_gsrc = 0;
   int* d_segcnt = nullptr;            // records per initialisation segment
   size_t cap_seg = 0;
+  unsigned char* d_skip = nullptr;    // per-task skip flags of the sparse heavy initialisation
+  size_t cap_skip = 0;
+  // reverse positions of the loaded graph (emst_load), when its lists are
+  // sorted and symmetric
+  int* d_rev = nullptr;
+  size_t cap_rev = 0;
+  bool rev_ok = false;
+  bool rev_mirrored = false;
   int* d_cnt = nullptr;
   long long* d_stats = nullptr;
   int* h_flag = nullptr;
@@ -82,6 +90,15 @@ struct WorkBufs {
   int* cnt = nullptr;           // [list 0 size, list 1 size, marked component, -]
   long long* stats = nullptr;   // ST_COUNT counters
   int* segcnt = nullptr;        // records per initialisation segment
+  unsigned char* skip = nullptr;  // per-task skip flags (sparse heavy initialisation)
+};
+
+// Reverse positions of a graph whose lists are sorted and symmetric: rev[p]
+// is the position of the upper-direction copy of the edge at lower-direction
+// position p.  They enable the sparse heavy initialisation.
+struct RevIndex {
+  const int* rev = nullptr;
+  bool mirrored = false;  // every edge's two copies carry the same weight
 };
 
 // Element counts of the work buffers for a solve of the given size under a
@@ -126,7 +143,15 @@ int ensure_buffers(emst_ctx* ctx, int32_t nodes, int32_t edges, long long wl_cap
 int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
               const int* d_eweight, int use_filter, int32_t threshold, uint8_t* d_inMST, hipStream_t s,
               long long wl_cap, const VertexThresholds* vt, emst_stats* stats, FixedSchedule* fixed = nullptr,
-              const WorkBufs* wb = nullptr);
+              const WorkBufs* wb = nullptr, const RevIndex* rix = nullptr);
+
+// Builds the reverse positions of a device-resident graph into rev (edges
+// ints) on stream s; *ok is set when the lists are sorted and every
+// lower-direction copy has a partner (the caller also checks that the
+// graph has as many lower- as upper-direction positions), *mirrored when
+// every pair carries one weight.
+int build_rev(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
+              const int* d_eweight, int* rev, hipStream_t s, bool* ok, bool* mirrored);
 
 // The reference's kernel sequence with switchable optimizations (the
 // EMST_F_LADDER and EMST_F_NO_* bits of the context's flags): one graph, a
@@ -143,6 +168,7 @@ int noalloc_setup(int device);
 // Flags that select the ladder schedule.
 constexpr int kLadderFlags = EMST_F_LADDER | EMST_F_NO_WARP | EMST_F_NO_IMPLICIT_PC | EMST_F_NO_ONE_DIRECTION |
                              EMST_F_NO_TUPLES | EMST_F_NO_DATA_DRIVEN | EMST_F_NO_EDGE_BASED;
-constexpr int kAllFlags = EMST_F_HOST_SYNC | EMST_F_NO_GUARD | EMST_F_NO_COMBINE | EMST_F_NO_PAIR_DEDUP | kLadderFlags;
+constexpr int kAllFlags = EMST_F_HOST_SYNC | EMST_F_NO_GUARD | EMST_F_NO_COMBINE | EMST_F_NO_PAIR_DEDUP |
+                          EMST_F_NO_SPARSE_HEAVY | kLadderFlags;
 
 }  // namespace emst

@@ -52,7 +52,7 @@ constexpr int kHflagSlots = 512;  // host-visible round results per solve
 constexpr int kSelItems = 4;      // records per thread per k_select tile
 constexpr int kTbl = 1024;        // LDS min-combining table entries
 constexpr int kPair = 1024;       // LDS parallel-record table entries (k_select after the first round)
-constexpr int kReportWords = ST_COUNT + 3;  // counters, threshold, filter flag, quantile bits
+constexpr int kReportWords = ST_COUNT + 4;  // counters, threshold, filter flag, quantile bits, sparse heavy
 
 // Dynamic LDS of an initialisation workgroup: per-wave staging and marks,
 // then the workgroup's fill count (16 bytes keep the size a multiple of 16).
@@ -75,6 +75,8 @@ __global__ __launch_bounds__(256) void k_setup(int nodes, const int* __restrict_
   if (v == 0) {
     cnt[0] = 0;
     cnt[1] = 0;
+    cnt[2] = 0;
+    cnt[3] = 0;  // the sparse heavy decision, set by k_pick_big
     for (int i = 0; i < ST_COUNT; i++) stats[i] = 0;
   }
   if (v < nodes) {
@@ -137,13 +139,116 @@ __global__ __launch_bounds__(256) void k_zero(uint8_t* __restrict__ p, long long
 // segcnt[b] and adds it to cnt[0].  seg_cap bounds the upper-direction edges
 // of the workgroup's tasks, so a segment never overflows.
 // ---------------------------------------------------------------------------
+// Vertex boundaries of a task, loaded before the next task's stream loads
+// are issued (so waiting for them does not wait for those): up to kMarkSpan
+// boundaries one per lane and row when the task's vertex range is short,
+// else a window of 64 for the search below.
+struct TaskBounds {
+  int stv[kMarkSpan / kWave];
+  int bnd;
+  bool by_marks;
+};
+
+__device__ __forceinline__ void task_bounds_load(TaskBounds& tb, const int* __restrict__ nindex, int nodes, int base,
+                                                 int vend, int lane)
+{
+  tb.by_marks = vend - base <= kMarkSpan;
+  tb.bnd = 0;
+  if (tb.by_marks) {
+#pragma unroll
+    for (int q = 0; q < kMarkSpan / kWave; q++) {
+      const long long v = (long long)base + 1 + q * kWave + lane;
+      tb.stv[q] = (q * kWave < vend - base && v <= vend) ? nindex[v] : INT_MAX;
+    }
+  } else {
+    const long long bi = (long long)base + 1 + lane;
+    tb.bnd = bi <= nodes ? nindex[bi] : INT_MAX;
+  }
+}
+
+// Source vertex of every position of the task starting at gbeg whose first
+// position belongs to vertex base: src(p) = max{v : nindex[v] <= p}.
+template <int SUB>
+__device__ __forceinline__ void task_sources(int (&src)[SUB], TaskBounds& tb, long long gbeg, int base, int edges,
+                                             int nodes, const int* __restrict__ nindex, int* marks, int lane)
+{
+  constexpr int TE = SUB * kWave;
+  if (tb.by_marks) {
+    // Every vertex v in (base, vend] whose adjacency starts inside the
+    // task writes v at its start offset (the largest v wins where
+    // zero-degree vertices share a start); a prefix maximum over the
+    // task's offsets then gives the source of every position.
+#pragma unroll
+    for (int u = 0; u < SUB; u++) marks[u * kWave + lane] = -1;
+    wave_lds_sync();
+#pragma unroll
+    for (int q = 0; q < kMarkSpan / kWave; q++) {
+      const long long off = (long long)tb.stv[q] - gbeg;
+      if (off > 0 && off < TE) atomicMax(&marks[off], base + 1 + q * kWave + lane);
+    }
+    wave_lds_sync();
+    // prefix maximum in position order: a wave max-scan per row of 64
+    // positions, carried from row to row
+    int carry = base;
+#pragma unroll
+    for (int u = 0; u < SUB; u++) {
+      const int x = max(wave_max_scan(marks[u * kWave + lane]), carry);
+      src[u] = x;
+      carry = __builtin_amdgcn_readlane(x, kWave - 1);
+    }
+    wave_lds_sync();  // the next task rewrites the marks
+  } else {
+    // long vertex ranges (runs of zero-degree vertices): search windows
+    // of 64 vertex boundaries held one per lane
+    int bnd = tb.bnd;
+#pragma unroll
+    for (int u = 0; u < SUB; u++) {
+      const long long pl = gbeg + u * kWave + lane;
+      const int p = pl < edges ? (int)pl : INT_MAX - 1;
+      bool done = pl >= edges;
+      int sv = 0;
+      for (;;) {
+        const int c = count_le(bnd, p);
+        const int b63 = __shfl(bnd, kWave - 1);
+        if (!done && p < b63) {
+          sv = base + c;
+          done = true;
+        }
+        if (__all(done)) break;
+        base += kWave;
+        const long long bi = (long long)base + 1 + lane;
+        bnd = bi <= nodes ? nindex[bi] : INT_MAX;
+      }
+      src[u] = sv;
+    }
+  }
+}
+
+// Ends an initialisation workgroup: flushes the wave's staging, then posts
+// the segment's record count and adds it to the list size and the counter.
+__device__ __forceinline__ void init_finish(Stage& st, int4* seg, int& s_fill, int* segcnt, int* cnt,
+                                            long long* stats, int stat)
+{
+  stage_flush(st, seg, &s_fill);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int f = s_fill;
+    segcnt[blockIdx.x] = f;
+    if (f) {
+      atomicAdd(cnt, f);
+      atomicAdd((unsigned long long*)&stats[stat], (unsigned long long)f);
+    }
+  }
+}
+
 template <bool HEAVY, int SUB>
 __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
     const int* __restrict__ nindex, const int* __restrict__ nlist, const int* __restrict__ eweight,
     const int* __restrict__ gsrc, int nodes, int edges, int T, const int* __restrict__ label,
     const unsigned* __restrict__ vclass, int4* __restrict__ wl, long long seg_cap, int* __restrict__ segcnt,
     int* __restrict__ cnt, int stage_cap, long long* __restrict__ stats, const int* __restrict__ dthr,
-    const int* __restrict__ vgid, const int* __restrict__ gthr, const int* __restrict__ bigroot_p)
+    const int* __restrict__ vgid, const int* __restrict__ gthr, const int* __restrict__ bigroot_p,
+    int sparse_ready)
 {
   constexpr int TE = SUB * kWave;
   // dynamic LDS: per-wave staging, per-wave marks, the workgroup's fill count
@@ -153,6 +258,8 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
     if (HEAVY && dthr[1] == 0) return;  // no heavy phase when unfiltered
     T = dthr[0];
   }
+  // the sparse pass was enqueued too and the marked component chose it
+  if (HEAVY && sparse_ready && bigroot_p[1]) return;
   if (threadIdx.x == 0) s_fill = 0;
   __syncthreads();
   int4* const seg = wl + (long long)blockIdx.x * seg_cap;
@@ -180,20 +287,8 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
   }
   for (; g < ngroups; g += nwaves) {
     const long long gbeg = g * TE;
-    const bool by_marks = vend - base <= kMarkSpan;
-    // vertex boundaries of this task (independent loads, issued first)
-    int stv[kMarkSpan / kWave];
-    long long bi = (long long)base + 1 + lane;
-    int bnd = 0;
-    if (by_marks) {
-#pragma unroll
-      for (int q = 0; q < kMarkSpan / kWave; q++) {
-        const long long v = (long long)base + 1 + q * kWave + lane;
-        stv[q] = (q * kWave < vend - base && v <= vend) ? nindex[v] : INT_MAX;
-      }
-    } else {
-      bnd = bi <= nodes ? nindex[bi] : INT_MAX;
-    }
+    TaskBounds tb;
+    task_bounds_load(tb, nindex, nodes, base, vend, lane);
     const long long gn = g + nwaves;
     int nl_n[SUB], w_n[SUB], base_n = 0, vend_n = 0;
     if (gn < ngroups) {
@@ -206,56 +301,8 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
         if (!HEAVY) w_n[u] = p < edges ? ld_stream(&eweight[p]) : 0;
       }
     }
-    // Source vertex of every position: src(p) = max{v : nindex[v] <= p}.
     int src[SUB];
-    if (by_marks) {
-      // Every vertex v in (base, vend] whose adjacency starts inside the
-      // task writes v at its start offset (the largest v wins where
-      // zero-degree vertices share a start); a prefix maximum over the
-      // task's offsets then gives the source of every position.
-#pragma unroll
-      for (int u = 0; u < SUB; u++) marks[u * kWave + lane] = -1;
-      wave_lds_sync();
-#pragma unroll
-      for (int q = 0; q < kMarkSpan / kWave; q++) {
-        const long long off = (long long)stv[q] - gbeg;
-        if (off > 0 && off < TE) atomicMax(&marks[off], base + 1 + q * kWave + lane);
-      }
-      wave_lds_sync();
-      // prefix maximum in position order: a wave max-scan per row of 64
-      // positions, carried from row to row
-      int carry = base;
-#pragma unroll
-      for (int u = 0; u < SUB; u++) {
-        const int x = max(wave_max_scan(marks[u * kWave + lane]), carry);
-        src[u] = x;
-        carry = __builtin_amdgcn_readlane(x, kWave - 1);
-      }
-      wave_lds_sync();  // the next task rewrites the marks
-    } else {
-      // long vertex ranges (runs of zero-degree vertices): search windows
-      // of 64 vertex boundaries held one per lane
-#pragma unroll
-      for (int u = 0; u < SUB; u++) {
-        const long long pl = gbeg + u * kWave + lane;
-        const int p = pl < edges ? (int)pl : INT_MAX - 1;
-        bool done = pl >= edges;
-        int sv = 0;
-        for (;;) {
-          const int c = count_le(bnd, p);
-          const int b63 = __shfl(bnd, kWave - 1);
-          if (!done && p < b63) {
-            sv = base + c;
-            done = true;
-          }
-          if (__all(done)) break;
-          base += kWave;
-          bi = (long long)base + 1 + lane;
-          bnd = bi <= nodes ? nindex[bi] : INT_MAX;
-        }
-        src[u] = sv;
-      }
-    }
+    task_sources<SUB>(src, tb, gbeg, base, edges, nodes, nindex, marks, lane);
     if (!HEAVY) {
 #pragma unroll
       for (int u = 0; u < SUB; u++) {
@@ -305,27 +352,239 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
       if (!HEAVY) w[u] = w_n[u];
     }
   }
-  stage_flush(st, seg, &s_fill);
+  init_finish(st, seg, s_fill, segcnt, cnt, stats, HEAVY ? ST_HEAVY : ST_LIGHT);
+}
+
+// ---------------------------------------------------------------------------
+// Sparse heavy initialisation (same output as k_init_wl<true>, for sorted,
+// symmetric lists with reverse positions).  A crossing edge has at least
+// one endpoint outside the marked component M, so only the positions of
+// sources outside M are examined, and a task whose source vertices all lie
+// in M is skipped without streaming it (skip[g], from k_task_skip).  For a
+// source v outside M and a position p of v with neighbour y:
+//   y > v (p is the upper copy): emitted when root(v) != root(y), as in
+//       k_init_wl<true>;
+//   y < v, y in M (p is the lower copy of an edge crossing out of M): the
+//       edge's upper copy is rev[p] in y's list, so the record is
+//       {root(y), root(v), eweight[rev[p]], rev[p]};
+//   y < v, y outside M: skipped; y's own walk emits that edge's upper copy.
+// Every crossing upper-direction edge is thus emitted exactly once, the
+// reference's initWL<false> set (ECL_MST.cu:157-207, launch :326).  With
+// mirrored weights eweight[p] equals eweight[rev[p]] and is streamed with
+// the task.  The wave's tasks are g0, g0 + nwaves, ...; their skip flags are
+// read 64 at a time (one per lane) and only the tasks to do are visited,
+// the next one's stream loads issued before the current one is processed.
+// ---------------------------------------------------------------------------
+template <int SUB>
+__global__ __launch_bounds__(kInitWaves * kWave) void k_init_heavy_sparse(
+    const int* __restrict__ nindex, const int* __restrict__ nlist, const int* __restrict__ eweight,
+    const int* __restrict__ gsrc, int nodes, int edges, const int* __restrict__ label,
+    const unsigned* __restrict__ vclass, const unsigned char* __restrict__ skip, const int* __restrict__ rev,
+    int mirrored, int4* __restrict__ wl, long long seg_cap, int* __restrict__ segcnt, int* __restrict__ cnt,
+    int stage_cap, long long* __restrict__ stats, const int* __restrict__ dthr, const int* __restrict__ bigroot_p)
+{
+  constexpr int TE = SUB * kWave;
+  extern __shared__ int4 lds_stage[];
+  int& s_fill = *(reinterpret_cast<int*>(lds_stage + kInitWaves * stage_cap) + kInitWaves * TE);
+  if (dthr && dthr[1] == 0) return;  // no heavy phase when unfiltered
+  if (!bigroot_p[1]) return;         // the full pass runs instead
+  if (threadIdx.x == 0) s_fill = 0;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const int f = s_fill;
-    segcnt[blockIdx.x] = f;
-    if (f) {
-      atomicAdd(cnt, f);
-      atomicAdd((unsigned long long*)&stats[HEAVY ? ST_HEAVY : ST_LIGHT], (unsigned long long)f);
+  int4* const seg = wl + (long long)blockIdx.x * seg_cap;
+  const int bigroot = *bigroot_p;
+  const int lane = lane_id();
+  const int wid = threadIdx.x / kWave;
+  Stage st{lds_stage + wid * stage_cap, stage_cap, 0};
+  int* marks = reinterpret_cast<int*>(lds_stage + kInitWaves * stage_cap) + wid * TE;
+  const long long ngroups = ((long long)edges + TE - 1) / TE;
+  const long long nwaves = (long long)gridDim.x * kInitWaves;
+  const long long g0 = (long long)blockIdx.x * kInitWaves + wid;
+  // the wave's tasks to do, 64 at a time: bit l of todo is task
+  // g0 + (64 * (chunk - 1) + l) * nwaves
+  long long chunk = 0;
+  u64 todo = 0;
+  auto next_task = [&]() -> long long {
+    for (;;) {
+      if (todo) {
+        const int l = __builtin_ctzll(todo);
+        todo &= todo - 1;
+        return g0 + (64 * (chunk - 1) + l) * nwaves;
+      }
+      if (g0 + 64 * chunk * nwaves >= ngroups) return -1;
+      const long long gl = g0 + (64 * chunk + lane) * nwaves;
+      todo = __ballot(gl < ngroups && !skip[gl]);
+      chunk++;
+    }
+  };
+  auto load_task = [&](long long t, int (&nl)[SUB], int (&wt)[SUB], int& base, int& vend) {
+    base = gsrc[t];
+    vend = t + 1 < ngroups ? gsrc[t + 1] : nodes - 1;
+#pragma unroll
+    for (int u = 0; u < SUB; u++) {
+      const long long p = t * TE + u * kWave + lane;
+      nl[u] = p < edges ? ld_stream(&nlist[p]) : -1;
+      wt[u] = (mirrored && p < edges) ? ld_stream(&eweight[p]) : 0;
+    }
+  };
+  long long g = next_task();
+  int nl[SUB], w[SUB], base = 0, vend = 0;
+  if (g >= 0) load_task(g, nl, w, base, vend);
+  while (g >= 0) {
+    const long long gbeg = g * TE;
+    TaskBounds tb;
+    task_bounds_load(tb, nindex, nodes, base, vend, lane);
+    const long long gn = next_task();
+    int nl_n[SUB], w_n[SUB], base_n = 0, vend_n = 0;
+    if (gn >= 0) load_task(gn, nl_n, w_n, base_n, vend_n);
+    int src[SUB];
+    task_sources<SUB>(src, tb, gbeg, base, edges, nodes, nindex, marks, lane);
+    // positions whose source lies outside M, and their neighbours' classes
+    unsigned cv[SUB], cu[SUB];
+    bool act[SUB];
+#pragma unroll
+    for (int u = 0; u < SUB; u++) {
+      const long long p = gbeg + u * kWave + lane;
+      cv[u] = (vclass[(unsigned)src[u] >> 4] >> ((src[u] & 15) * 2)) & 3u;
+      act[u] = p < edges && !(cv[u] & 1u) && nl[u] != src[u];
+      cu[u] = act[u] ? (vclass[(unsigned)nl[u] >> 4] >> ((nl[u] & 15) * 2)) & 3u : 0u;
+    }
+    int a[SUB], b[SUB];
+    bool keep[SUB];
+#pragma unroll
+    for (int u = 0; u < SUB; u++) {
+      const bool upper = nl[u] > src[u];
+      keep[u] = act[u] && (upper || (cu[u] & 1u));
+      a[u] = (cv[u] & 2u) ? src[u] : -1;
+      b[u] = (cu[u] & 1u) ? bigroot : (cu[u] & 2u) ? nl[u] : -1;
+      if (keep[u] && a[u] < 0) a[u] = label[src[u]];
+      if (keep[u] && b[u] < 0) b[u] = label[nl[u]];
+      keep[u] = keep[u] && a[u] != b[u];
+    }
+#pragma unroll
+    for (int u = 0; u < SUB; u++) {
+      const long long p = gbeg + u * kWave + lane;
+      const bool lower = nl[u] < src[u];
+      int j = (int)p, wv = w[u];
+      if (keep[u] && lower) j = rev[p];
+      if (keep[u] && !mirrored) wv = eweight[j];
+      // records list the lower endpoint's root first, as k_init_wl<true>
+      const int4 rec = lower ? make_int4(b[u], a[u], wv, j) : make_int4(a[u], b[u], wv, j);
+      stage_push(st, keep[u], rec, seg, &s_fill);
+    }
+    g = gn;
+    base = base_n;
+    vend = vend_n;
+#pragma unroll
+    for (int u = 0; u < SUB; u++) {
+      nl[u] = nl_n[u];
+      w[u] = w_n[u];
+    }
+  }
+  init_finish(st, seg, s_fill, segcnt, cnt, stats, ST_HEAVY);
+}
+
+// Skip flags of the sparse heavy initialisation: task g (positions
+// g*task_edges ...) is skipped when every vertex in [gsrc[g], gsrc[g+1]]
+// (a superset of its sources) lies in the marked component (bit 0 of its
+// class).  Ranges longer than 4096 vertices are not examined and not
+// skipped.
+__global__ __launch_bounds__(256) void k_task_skip(long long ngroups, int nodes, const int* __restrict__ gsrc,
+                                                   const unsigned* __restrict__ vclass, unsigned char* __restrict__ skip,
+                                                   const int* __restrict__ dthr, const int* __restrict__ sparse_on)
+{
+  if ((dthr && dthr[1] == 0) || !*sparse_on) return;
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngroups) return;
+  const int lo = gsrc[g];
+  const int hi = g + 1 < ngroups ? gsrc[g + 1] : nodes - 1;
+  bool all = hi - lo <= 4096;
+  for (int v = lo; all && v <= hi; v++) all = (vclass[(unsigned)v >> 4] >> ((v & 15) * 2)) & 1u;
+  skip[g] = all ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Reverse positions, built once per loaded graph (emst_load): for a
+// lower-direction position p of vertex x (y = nlist[p] < x), rev[p] is the
+// position of the same edge's upper-direction copy in y's list: the k-th copy
+// of y in x's list pairs with the k-th copy of x in y's list, found by a
+// binary search (lists must be sorted).  bad[0] counts positions out of
+// order, bad[1] lower copies without such a partner, bad[2] partners of
+// another weight.  With no position out of order, every lower copy
+// partnered and as many lower as upper copies, the pairing is a bijection
+// and the sparse heavy initialisation is exact.  starts[t] is the source
+// of position t*kRevTask, which narrows the search for a position's source.
+// ---------------------------------------------------------------------------
+constexpr int kRevTask = 256;
+
+__global__ __launch_bounds__(256) void k_rev_starts(int nodes, const int* __restrict__ nindex, int* __restrict__ starts)
+{
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nodes) return;
+  const int beg = nindex[v], end = nindex[v + 1];
+  for (long long t = ((long long)beg + kRevTask - 1) / kRevTask; t < ((long long)end + kRevTask - 1) / kRevTask; t++)
+    starts[t] = (int)v;
+}
+
+__global__ __launch_bounds__(256) void k_rev(int nodes, int edges, const int* __restrict__ nindex,
+                                             const int* __restrict__ nlist, const int* __restrict__ eweight,
+                                             const int* __restrict__ starts, int* __restrict__ rev,
+                                             unsigned long long* __restrict__ bad)
+{
+  const long long ntasks = ((long long)edges + kRevTask - 1) / kRevTask;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < edges; p += stride) {
+    const long long t = p / kRevTask;
+    int lo = starts[t], hi = t + 1 < ntasks ? starts[t + 1] : nodes - 1;
+    while (lo < hi) {  // the source: the largest v with nindex[v] <= p
+      const int mid = lo + (hi - lo + 1) / 2;
+      if (nindex[mid] <= p) lo = mid;
+      else hi = mid - 1;
+    }
+    const int x = lo, y = nlist[p], first = nindex[x];
+    if (p > first && nlist[p - 1] > y) atomicAdd(&bad[0], 1ull);
+    if (y >= x) continue;
+    int k = 0;
+    for (long long q = p - 1; q >= first && nlist[q] == y; q--) k++;
+    int a = nindex[y], b = nindex[y + 1];
+    const int yend = b;
+    while (a < b) {  // first position of x in y's list
+      const int mid = a + (b - a) / 2;
+      if (nlist[mid] < x) a = mid + 1;
+      else b = mid;
+    }
+    const long long r = (long long)a + k;
+    if (r < yend && nlist[r] == x) {
+      rev[p] = (int)r;
+      if (eweight[r] != eweight[p]) atomicAdd(&bad[2], 1ull);
+    } else {
+      rev[p] = -1;
+      atomicAdd(&bad[1], 1ull);
     }
   }
 }
 
 // Picks the component that the heavy pass' class map marks: the most frequent
-// root among 64 evenly spaced vertices (one wave).  Any choice is correct;
-// a frequent one saves the most root lookups.
-__global__ __launch_bounds__(64) void k_pick_big(int nodes, const int* __restrict__ parent, int* __restrict__ out,
+// root among the targets of 64 evenly spaced positions (one wave), so the
+// vote weighs components by their adjacency (vertices when there are no
+// edges).  Any choice is correct; one holding most positions saves the most
+// root lookups, and lets the sparse heavy initialisation skip the most.
+// out[0] = its root; out[1] = 1 when it holds at least half of the sampled
+// positions, which selects the sparse heavy initialisation where it is
+// available (it skips the component's own adjacency, so it pays only when
+// that adjacency dominates).
+__global__ __launch_bounds__(64) void k_pick_big(int nodes, int edges, const int* __restrict__ nlist,
+                                                 const int* __restrict__ parent, int* __restrict__ out,
                                                  const int* __restrict__ dthr)
 {
   if (dthr && dthr[1] == 0) return;
   const int lane = lane_id();
-  const int v = (int)(((long long)lane * nodes) / kWave + ((long long)nodes / (2 * kWave)));
+  int v;
+  if (edges > 0) {
+    const long long p = ((long long)lane * edges) / kWave + (long long)edges / (2 * kWave);
+    v = nlist[p < edges ? p : edges - 1];
+  } else {
+    v = (int)(((long long)lane * nodes) / kWave + ((long long)nodes / (2 * kWave)));
+  }
   const int r = find_root(v < nodes ? v : nodes - 1, parent);
   int c = 0;
   for (int k = 0; k < kWave; k++) c += __shfl(r, k) == r;
@@ -334,7 +593,10 @@ __global__ __launch_bounds__(64) void k_pick_big(int nodes, const int* __restric
   for (int off = 32; off >= 1; off >>= 1) best = max(best, __shfl_xor(best, off));
   const int winner = kWave - 1 - (best & 0xff);
   const int big = __shfl(r, winner);
-  if (lane == 0) out[0] = big;
+  if (lane == 0) {
+    out[0] = big;
+    out[1] = edges > 0 && 2 * (best >> 8) >= kWave;
+  }
 }
 
 // Spreads the 16 low bits of x to the even bit positions of a word.
@@ -454,19 +716,45 @@ struct ListView {
   int nseg;
 };
 
-// Positions below the returned bound hold records, for the tile (or row
-// group) starting at physical position base.
-__device__ __forceinline__ long long list_limit(const ListView& lv, long long base, long long n_dense)
+// Walks the tiles of a list that belong to this workgroup and calls
+// fn(base, lim) for each: base is the tile's first physical position, lim
+// the end of its segment's records.  A segmented list hands whole segments
+// to workgroups (one count load per segment and no division per tile; with
+// more workgroups than segments, a segment's tiles are split among several);
+// a dense list is the one-segment case, walked grid-stride.  The walk is
+// uniform across the workgroup.
+template <class F>
+__device__ __forceinline__ void walk_tiles(const ListView& lv, long long n_dense, long long tile, F&& fn)
 {
-  if (!lv.segcnt) return n_dense;
-  const long long sidx = base / lv.seg_cap;
-  return sidx * lv.seg_cap + lv.segcnt[sidx];
+  const int P = gridDim.x, w = blockIdx.x;
+  const int nseg = lv.segcnt ? lv.nseg : 1;
+  int s0, sstep;
+  long long j0, jstep;
+  if (P <= nseg) {
+    s0 = w;
+    sstep = P;
+    j0 = 0;
+    jstep = 1;
+  } else {
+    const int k = P / nseg;
+    s0 = w < k * nseg ? w % nseg : nseg;
+    sstep = nseg;
+    j0 = w / nseg;
+    jstep = k;
+  }
+  for (int sg = s0; sg < nseg; sg += sstep) {
+    const long long c = lv.segcnt ? (long long)lv.segcnt[sg] : n_dense;
+    const long long sb = (long long)sg * lv.seg_cap;
+    for (long long j = j0; j * tile < c; j += jstep) fn(sb + j * tile, sb + c);
+  }
 }
 
+// Physical positions a list spans (the size of its bitmasks).
 __device__ __forceinline__ long long list_span(const ListView& lv, long long n_dense)
 {
   return lv.segcnt ? (long long)lv.nseg * lv.seg_cap : n_dense;
 }
+
 __device__ __forceinline__ int pair_slot(u64* tags, u64* keys, u64 pair, u64 key)
 {
   const unsigned h = ((unsigned)(pair ^ (pair >> 29)) * 0x9E3779B1u) >> (32 - 10);
@@ -523,11 +811,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_select(
   __syncthreads();
   const int lane = lane_id();
   const int wid = tid / kWave;
-  const long long ntiles = (span + TILE - 1) / TILE;
-  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const long long base = t * TILE;
-    const long long lim = list_limit(lv, base, n_in);
-    if (base >= lim) continue;  // an empty part of a segment (uniform per tile)
+  walk_tiles(lv, n_in, TILE, [&](const long long base, const long long lim) {
     int4 r[kSelItems];
     bool live[kSelItems];
 #pragma unroll
@@ -568,7 +852,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_select(
       }
       mine += keep[k];
     }
-    if (FIRST) continue;
+    if (FIRST) return;
     if (dedup) {
       // keep the lightest record of each root pair in this tile
       int slot[kSelItems];
@@ -620,7 +904,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_select(
     for (int k = 0; k < kSelItems; k++)
       if (keep[k]) wl_out[pos++] = r[k];
     __syncthreads();
-  }
+  });
   // publish the workgroup's combined minima (after every offer of every
   // wave: the first-round pass has no per-tile barrier)
   __syncthreads();
@@ -728,11 +1012,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_join(
   if (n == 0) return;  // (segment counts are only valid for a non-empty list)
   const long long span = list_span(lv, n);
   const long long cand_words = (span + kWave - 1) / kWave;
-  const long long ntiles = (span + BT - 1) / BT;
-  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const long long base = t * BT;
-    const long long lim = list_limit(lv, base, n);
-    if (base >= lim) continue;
+  walk_tiles(lv, n, BT, [&](const long long base, const long long lim) {
     const long long i = base + threadIdx.x;
     const bool live = i < lim;
     // after the offer-only first round, only a candidate endpoint can have
@@ -763,27 +1043,32 @@ __global__ __launch_bounds__(WAVES * kWave) void k_join(
     // Rows that start at or past the limit hold no record and write no word.
     const u64 jm = __ballot(joined);
     if (lane_id() == 0 && i < lim) joined_mask[i / kWave] = jm;
-  }
+  });
 }
 
 // Copies the solve's counters, threshold and quantile into host-mapped
 // memory with system-scope stores, so the host reads them after the final
 // event without further copies.
 __global__ __launch_bounds__(64) void k_report(const long long* __restrict__ stats, const int* __restrict__ thr,
-                                               const float* __restrict__ q, long long* __restrict__ out)
+                                               const float* __restrict__ q, const int* __restrict__ sparse,
+                                               long long* __restrict__ out)
 {
   const int t = threadIdx.x;
   long long v = 0;
   if (t < ST_COUNT) v = stats[t];
   else if (t < ST_COUNT + 2) v = thr ? thr[t - ST_COUNT] : 0;
   else if (t == ST_COUNT + 2) v = q ? (long long)__float_as_uint(q[0]) : 0;
+  else if (t == ST_COUNT + 3) v = sparse ? *sparse : 0;
   if (t < kReportWords) __hip_atomic_store(&out[t], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // ---- runtime selection of template instances -------------------------------
 typedef void (*InitFn)(const int*, const int*, const int*, const int*, int, int, int, const int*,
                        const unsigned*, int4*, long long, int*, int*, int, long long*, const int*, const int*,
-                       const int*, const int*);
+                       const int*, const int*, int);
+typedef void (*SparseFn)(const int*, const int*, const int*, const int*, int, int, const int*, const unsigned*,
+                         const unsigned char*, const int*, int, int4*, long long, int*, int*, int, long long*,
+                         const int*, const int*);
 typedef void (*SelFn)(const int4*, const int*, int4*, int*, const int*, u64*, int, int, int, long long*, int, u64*,
                       const u64*, ListView);
 typedef void (*JoinFn)(int4*, int*, int4*, int*, int*, u64*, uint8_t*, int, int*, long long*, int, const u64*, u64*,
@@ -799,6 +1084,13 @@ InitFn init_fn(bool heavy, int sub)
   if (sub == 4) return k_init_wl<false, 4>;
   if (sub == 16) return k_init_wl<false, 16>;
   return k_init_wl<false, 8>;
+}
+
+SparseFn sparse_fn(int sub)
+{
+  if (sub == 4) return k_init_heavy_sparse<4>;
+  if (sub == 16) return k_init_heavy_sparse<16>;
+  return k_init_heavy_sparse<8>;
 }
 
 SelFn sel_fn(bool first, int waves)
@@ -856,10 +1148,10 @@ emst_config default_config()
   c.block = 512;
   c.grid_per_cu = 4;
   c.tile_edges = 512;
-  c.stage_recs = 512;
+  c.stage_recs = 256;
   c.tail_max = 4096;
   c.flags = 0;
-  c.init_grid_per_cu = 4;
+  c.init_grid_per_cu = 5;
   return c;
 }
 
@@ -931,6 +1223,40 @@ int ensure_buffers(emst_ctx* ctx, int32_t nodes, int32_t edges, long long wl_cap
   if ((rc = grow((void**)&ctx->d_wl[1], &ctx->cap_wl[1], z.wl[1], sizeof(int4), "worklist 1"))) return rc;
   if ((rc = grow((void**)&ctx->d_cand, &ctx->cap_cand, 2 * z.cand_words, 8, "candidate bitmasks"))) return rc;
   if ((rc = grow((void**)&ctx->d_segcnt, &ctx->cap_seg, z.segs, 4, "segment counts"))) return rc;
+  if ((rc = grow((void**)&ctx->d_skip, &ctx->cap_skip, z.gsrc, 1, "task skip flags"))) return rc;
+  return EMST_OK;
+}
+
+int build_rev(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
+              const int* d_eweight, int* rev, hipStream_t s, bool* ok, bool* mirrored)
+{
+  *ok = false;
+  *mirrored = false;
+  if (edges <= 0) return EMST_OK;
+  const long long ntasks = ((long long)edges + kRevTask - 1) / kRevTask;
+  int* starts = nullptr;
+  unsigned long long* bad = nullptr;
+  if (hipMalloc((void**)&starts, (size_t)ntasks * 4) != hipSuccess) return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc failed");
+  if (hipMalloc((void**)&bad, 3 * 8) != hipSuccess) {
+    (void)hipFree(starts);
+    return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc failed");
+  }
+  unsigned long long hb[3] = {1, 1, 1};
+  bool fine = hipMemsetAsync(bad, 0, 3 * 8, s) == hipSuccess;
+  if (fine) {
+    hipLaunchKernelGGL(k_rev_starts, dim3((nodes + 255) / 256), dim3(256), 0, s, nodes, d_nindex, starts);
+    const int grid = (int)std::min<long long>(((long long)edges + 255) / 256, (long long)ctx->cus * 32);
+    hipLaunchKernelGGL(k_rev, dim3(grid), dim3(256), 0, s, nodes, edges, d_nindex, d_nlist, d_eweight, starts, rev,
+                       bad);
+    fine = hipGetLastError() == hipSuccess &&
+           hipMemcpyAsync(hb, bad, sizeof(hb), hipMemcpyDeviceToHost, s) == hipSuccess &&
+           hipStreamSynchronize(s) == hipSuccess;
+  }
+  (void)hipFree(starts);
+  (void)hipFree(bad);
+  if (!fine) return ctx_fail(ctx, EMST_E_HIP, "building reverse positions failed");
+  *ok = hb[0] == 0 && hb[1] == 0;
+  *mirrored = hb[2] == 0;
   return EMST_OK;
 }
 
@@ -941,7 +1267,7 @@ namespace emst {
 int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
               const int* d_eweight, int use_filter, int32_t threshold, uint8_t* d_inMST, hipStream_t s,
               long long wl_cap, const VertexThresholds* vt, emst_stats* stats, FixedSchedule* fixed,
-              const WorkBufs* wbp)
+              const WorkBufs* wbp, const RevIndex* rix)
 {
   if (nodes < 1 || edges < 0) return ctx_fail(ctx, EMST_E_ARG, "node or edge count too low");
   if (use_filter < 0 || use_filter > EMST_FILTER_LEARNED) return ctx_fail(ctx, EMST_E_ARG, "bad filter mode");
@@ -976,7 +1302,10 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
     wb.cnt = ctx->d_cnt;
     wb.stats = ctx->d_stats;
     wb.segcnt = ctx->d_segcnt;
+    wb.skip = ctx->d_skip;
   }
+  // the sparse heavy initialisation needs reverse positions and skip flags
+  const bool sparse = rix && rix->rev && wb.skip && !(cfg.flags & EMST_F_NO_SPARSE_HEAVY);
   const WorkSizes z = work_sizes(ctx->cus, cfg, nodes, edges, wl_cap);
   const bool learned = use_filter == EMST_FILTER_LEARNED && !vt;
   const int T = use_filter == EMST_FILTER_PLAIN ? threshold : INT_MAX;
@@ -1046,17 +1375,31 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
     }
     if (phase == 1) {
       const int blocks = (int)(((long long)nodes + 255) / 256);
-      hipLaunchKernelGGL(k_pick_big, dim3(1), dim3(kWave), 0, s, nodes, wb.parent, &d_cnt[2], dthr);
+      hipLaunchKernelGGL(k_pick_big, dim3(1), dim3(kWave), 0, s, nodes, edges, d_nlist, wb.parent, &d_cnt[2], dthr);
       hipLaunchKernelGGL(k_flatten, dim3(blocks), dim3(256), 0, s, nodes, wb.parent, d_cnt, &d_cnt[2], wb.vclass,
                          dthr);
+      if (sparse && edges > 0) {
+        const long long ngroups = ((long long)edges + task_edges - 1) / task_edges;
+        hipLaunchKernelGGL(k_task_skip, dim3((unsigned)((ngroups + 255) / 256)), dim3(256), 0, s, ngroups, nodes,
+                           wb.gsrc, wb.vclass, wb.skip, dthr, &d_cnt[3]);
+      }
     }
     if (edges > 0) {
       if (!fixed) HIPCHK(hipEventRecord(ctx->ev[2 + 2 * phase], s));
+      // With reverse positions both heavy passes are enqueued; the marked
+      // component's share of the adjacency (k_pick_big) lets one of them run
+      // and the other return at once.
+      if (phase == 1 && sparse) {
+        const SparseFn f = sparse_fn(sub);
+        hipLaunchKernelGGL(f, dim3(z.init_grid), dim3(kInitWaves * kWave), init_lds, s, d_nindex, d_nlist, d_eweight,
+                           wb.gsrc, nodes, edges, wb.parent, wb.vclass, wb.skip, rix->rev, rix->mirrored ? 1 : 0,
+                           wl[0], z.seg_cap, wb.segcnt, &d_cnt[0], stage_cap, d_stats, dthr, &d_cnt[2]);
+      }
       const InitFn f = init_fn(phase == 1, sub);
       hipLaunchKernelGGL(f, dim3(z.init_grid), dim3(kInitWaves * kWave), init_lds, s, d_nindex, d_nlist, d_eweight,
                          wb.gsrc, nodes, edges, T, wb.parent, wb.vclass, wl[0], z.seg_cap, wb.segcnt, &d_cnt[0],
                          stage_cap, d_stats, dthr, phase == 0 ? vgid : nullptr, phase == 0 ? gthr : nullptr,
-                         &d_cnt[2]);
+                         &d_cnt[2], (phase == 1 && sparse) ? 1 : 0);
       if (!fixed) HIPCHK(hipEventRecord(ctx->ev[3 + 2 * phase], s));
       HIPCHK(hipGetLastError());
       init_launches++;
@@ -1073,17 +1416,20 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       // first select is the offer-only pass over the initial list
       const int cur = r & 1, nxt = cur ^ 1;
       const ListView lv = r == 0 ? segs : dense;
+      // one workgroup per segment over the segmented list (segments are
+      // filled alike, so this balances the work)
+      const int grid = r == 0 ? z.init_grid : round_grid;
       ctx->h_flag[slot] = -1;
       if (r == 0)
-        hipLaunchKernelGGL(sel_first, dim3(round_grid), dim3(waves * kWave), 0, s, wl[0], &d_cnt[0], wl[1], &d_cnt[1],
+        hipLaunchKernelGGL(sel_first, dim3(grid), dim3(waves * kWave), 0, s, wl[0], &d_cnt[0], wl[1], &d_cnt[1],
                            wb.parent, wb.minv, guard, combine, dedup, d_stats, rounds_slot, wb.cand,
                            (const u64*)nullptr, lv);
       // the first join reads the candidate bitmasks; every join marks what it joined
-      hipLaunchKernelGGL(join, dim3(round_grid), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt], &d_cnt[nxt],
+      hipLaunchKernelGGL(join, dim3(grid), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt], &d_cnt[nxt],
                          wb.parent, wb.minv, d_inMST, tail_max, ctx->d_flag + slot, d_stats, rounds_slot,
                          (const u64*)(r == 0 ? wb.cand : nullptr), wb.cand, lv);
       // each select drops the records the join before it joined
-      hipLaunchKernelGGL(sel, dim3(round_grid), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt], &d_cnt[nxt],
+      hipLaunchKernelGGL(sel, dim3(grid), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt], &d_cnt[nxt],
                          wb.parent, wb.minv, guard, combine, dedup, d_stats, rounds_slot, (u64*)nullptr,
                          (const u64*)wb.cand, lv);
       HIPCHK(hipGetLastError());
@@ -1113,7 +1459,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
   if (fixed) return EMST_OK;  // captured: the caller times and reads back
   if (stats) {
     hipLaunchKernelGGL(k_report, dim3(1), dim3(64), 0, s, d_stats, learned ? ctx->d_thr : nullptr,
-                       learned ? ctx->d_q : nullptr, ctx->d_report);
+                       learned ? ctx->d_q : nullptr, sparse ? &d_cnt[3] : nullptr, ctx->d_report);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(ctx->ev[1], s));
@@ -1153,6 +1499,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
     stats->tail_rounds = (int32_t)hs[ST_TAIL];
     stats->round_in_sum = hs[ST_IN_SUM];
     stats->round_out_sum = hs[ST_OUT_SUM];
+    stats->sparse_heavy = (int32_t)hs[ST_COUNT + 3];
   }
   return EMST_OK;
 }
@@ -1195,10 +1542,13 @@ int emst_create(int device, emst_ctx** out)
        hipMalloc((void**)&ctx->d_samples, kWeightSamples * 4) == hipSuccess;
   if (ok) {
     const int dyn = (int)ctx->lds_per_block;
-    for (int h = 0; h < 2 && ok; h++)
-      for (int sub : {4, 8, 16})
+    for (int sub : {4, 8, 16}) {
+      for (int h = 0; h < 2 && ok; h++)
         ok = ok && hipFuncSetAttribute((const void*)init_fn(h == 1, sub), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        dyn) == hipSuccess;
+      ok = ok && hipFuncSetAttribute((const void*)sparse_fn(sub), hipFuncAttributeMaxDynamicSharedMemorySize, dyn) ==
+                     hipSuccess;
+    }
   }
   if (!ok) {
     emst_destroy(ctx);
@@ -1214,6 +1564,7 @@ void emst_destroy(emst_ctx* ctx)
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   void* ptrs[] = {ctx->d_cand,   ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, ctx->d_inMST, ctx->d_parent, ctx->d_segcnt,
+                  ctx->d_skip,   ctx->d_rev,
                   ctx->d_minv,   ctx->d_wl[0], ctx->d_wl[1],   ctx->d_gsrc,  ctx->d_cnt,
                   ctx->d_stats,  ctx->d_vclass, ctx->d_mlp16,  ctx->d_thr,   ctx->d_q,
                   ctx->d_feat,   ctx->d_samples, ctx->d_lad[0], ctx->d_lad[1], ctx->d_slist,
@@ -1300,15 +1651,43 @@ int emst_load(emst_ctx* ctx, const emst_graph* g)
   }
   // Worklist capacity: the number of upper-direction edges (the reference
   // assumes a symmetric input and allocates edges/2, ECL_MST.cu:279, 285).
-  std::vector<long long> part((size_t)emst::host_threads(), 0);
+  // Lower-direction positions are counted too: reverse positions need as
+  // many of them as upper-direction ones.
+  const int nt = emst::host_threads();
+  std::vector<long long> part((size_t)nt, 0), part_lo((size_t)nt, 0);
   emst::parallel_for(g->nodes, [&](int64_t b, int64_t e, int t) {
-    long long c = 0;
+    long long c = 0, lo = 0;
     for (int64_t v = b; v < e; v++)
-      for (int32_t j = g->nindex[v]; j < g->nindex[v + 1]; j++) c += g->nlist[j] > v;
+      for (int32_t j = g->nindex[v]; j < g->nindex[v + 1]; j++) {
+        c += g->nlist[j] > v;
+        lo += g->nlist[j] < v;
+      }
     part[t] = c;
-  }, (int)part.size());
-  long long up = 0;
-  for (long long c : part) up += c;
+    part_lo[t] = lo;
+  }, nt);
+  long long up = 0, low = 0;
+  for (int t = 0; t < nt; t++) {
+    up += part[t];
+    low += part_lo[t];
+  }
+  ctx->loaded = false;
+  ctx->rev_ok = false;
+  ctx->rev_mirrored = false;
+  if (up == low && ne > 0) {
+    // reverse positions for the sparse heavy initialisation (a sorted,
+    // symmetric CSR: the generators' output and the reference's inputs)
+    if (ne > ctx->cap_rev || !ctx->d_rev) {
+      if (ctx->d_rev) (void)hipFree(ctx->d_rev);
+      ctx->d_rev = nullptr;
+      ctx->cap_rev = 0;
+      if (hipMalloc((void**)&ctx->d_rev, ne * 4) != hipSuccess)
+        return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of the reverse positions failed");
+      ctx->cap_rev = ne;
+    }
+    int rc2 = build_rev(ctx, g->nodes, g->edges, ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, ctx->d_rev, ctx->stream,
+                        &ctx->rev_ok, &ctx->rev_mirrored);
+    if (rc2) return rc2;
+  }
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->n = g->nodes;
   ctx->m = g->edges;
@@ -1322,8 +1701,13 @@ int emst_solve(emst_ctx* ctx, int use_filter, int32_t threshold, emst_stats* sta
   if (!ctx) return EMST_E_ARG;
   if (!ctx->loaded) return ctx_fail(ctx, EMST_E_STATE, "emst_solve before emst_load");
   HIPCHK(hipSetDevice(ctx->device));
+  RevIndex rix;
+  if (ctx->rev_ok) {
+    rix.rev = ctx->d_rev;
+    rix.mirrored = ctx->rev_mirrored;
+  }
   return run_solve(ctx, ctx->n, ctx->m, ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, use_filter, threshold,
-                   ctx->d_inMST, ctx->stream, ctx->m_up, nullptr, stats);
+                   ctx->d_inMST, ctx->stream, ctx->m_up, nullptr, stats, nullptr, nullptr, &rix);
 }
 
 int emst_fetch(emst_ctx* ctx, uint8_t* inMST)

@@ -305,3 +305,71 @@ def test_parallel_edge_multigraphs(solver):
         g = G(*csr_from_edges(n, edges))
         for uf, T in ((True, 2), (True, 4), (False, emst.INT_MAX)):
             check(solver, g, uf, T)
+
+
+def _sparse_cases():
+    rng = np.random.default_rng(23)
+    out = [("rmat14", emst.gen_rmat(14, 16, 0.57, 0.19, 0.19, 2, 1, 1 << 24)),
+           ("rmat13-ties", emst.gen_rmat(13, 16, 0.57, 0.19, 0.19, 5, 1, 9)),
+           ("grid", emst.gen_grid(90, 70, 4, 1, 50)),
+           ("rgg", emst.gen_rgg(20000, 6.0, 3, 1 << 20)),
+           ("er", emst.gen_er(20000, 10.0, 4, 1, 1 << 20))]
+    for i in range(12):
+        n = int(rng.integers(2, 60))
+        out.append((f"multi{i}", G(*random_graph(rng, n, int(rng.integers(0, 200)), 0, 4, allow_loops=True))))
+    return out
+
+
+def test_sparse_heavy_matches_full_pass(solver):
+    """The sparse heavy initialisation (sorted, symmetric lists: only the
+    adjacency outside the marked component is examined, crossing edges out
+    of it found through reverse positions) emits the full pass's records:
+    same bytes, L, H and rounds as with EMST_F_NO_SPARSE_HEAVY and as the
+    oracle, for the plain rule and for thresholds that make the light phase
+    small or large."""
+    ran = 0
+    for name, g in _sparse_cases():
+        uf, T = emst.plain_threshold(g)
+        w = np.sort(g.eweight) if g.edges else np.zeros(1, np.int32)
+        for T2 in {T, int(w[len(w) // 50]), int(w[len(w) // 2]), int(w[-1]) - 1}:
+            got, st = check(solver, g, True, T2, rounds=False)
+            got2, st2 = with_flags(solver, emst.F_NO_SPARSE_HEAVY, lambda: solver.gpu_mst(g, True, T2))
+            assert np.array_equal(got, got2), name
+            assert (st.light, st.heavy, st.light_rounds, st.heavy_rounds) == \
+                (st2.light, st2.heavy, st2.light_rounds, st2.heavy_rounds), name
+            assert st2.sparse_heavy == 0
+            ran += st.sparse_heavy
+    assert ran >= 20  # the sparse pass ran wherever the marked component holds most positions
+
+
+def test_sparse_heavy_unmirrored_weights(solver):
+    """Symmetric, sorted lists whose two copies of an edge carry different
+    weights: the reference keys an edge by its upper-direction copy's weight,
+    and so does the sparse pass (it reads eweight at the reverse position)."""
+    rng = np.random.default_rng(31)
+    for it in range(8):
+        ni, nl, ew = random_graph(rng, 300, 1500, 1, 1000)
+        src = np.repeat(np.arange(300), np.diff(ni))
+        low = nl < src
+        ew = ew.copy()
+        ew[low] = rng.integers(1, 1000, int(low.sum()))
+        g = G(ni, nl, ew)
+        for T in (50, 300):
+            got, _ = check(solver, g, True, T, rounds=False)
+            got2, _ = with_flags(solver, emst.F_NO_SPARSE_HEAVY, lambda: solver.gpu_mst(g, True, T))
+            assert np.array_equal(got, got2)
+
+
+def test_sparse_heavy_falls_back_on_unsorted_or_one_way_lists(solver):
+    """Unsorted lists or one-way edges have no reverse positions: the full
+    heavy pass runs and the result is still the oracle's."""
+    rng = np.random.default_rng(37)
+    for it in range(6):
+        n = 200
+        edges = [(int(a), int(b), int(c)) for a, b, c in zip(rng.integers(0, n, 700), rng.integers(0, n, 700),
+                                                             rng.integers(1, 30, 700))]
+        g1 = G(*csr_from_edges(n, edges, symmetric=True, sort=False))
+        g2 = G(*csr_from_edges(n, edges, symmetric=False))
+        for g in (g1, g2):
+            for T in (3, 12):
+                check(solver, g, True, T, rounds=False, kruskal=g is g1)
