@@ -48,6 +48,8 @@ namespace {
 
 constexpr int kInitWaves = 4;     // waves per workgroup in k_init_wl
 constexpr int kMarkSpan = 256;   // longest vertex range of a task resolved by marks
+constexpr int kStartEdges = 256;  // granularity of the task start table (the smallest task)
+constexpr int kSparseSub = 4;     // 256-edge tasks in the sparse heavy initialisation
 constexpr int kHflagSlots = 512;  // host-visible round results per solve
 constexpr int kSelItems = 4;      // records per thread per k_select tile
 constexpr int kTbl = 1024;        // LDS min-combining table entries
@@ -62,13 +64,14 @@ size_t init_lds_bytes(int stage_recs, int task_edges)
 }
 
 // ---------------------------------------------------------------------------
-// k_setup: initPM (ECL_MST.cu:147-154) plus the start vertex of every
-// task_edges-edge task: the vertex v whose adjacency range holds the task's
-// first edge writes it.  Zero-degree vertices own no edge and write nothing.
+// k_setup: initPM (ECL_MST.cu:147-154) plus the task start table: for every
+// kStartEdges-edge block, the vertex v whose adjacency range holds the
+// block's first edge writes it (a task of k blocks reads every k-th entry).
+// Zero-degree vertices own no edge and write nothing.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_setup(int nodes, const int* __restrict__ nindex,
                                                int* __restrict__ parent, u64* __restrict__ minv,
-                                               int* __restrict__ gsrc, int task_edges, int* __restrict__ cnt,
+                                               int* __restrict__ gsrc, int* __restrict__ cnt,
                                                long long* __restrict__ stats)
 {
   const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -84,8 +87,8 @@ __global__ __launch_bounds__(256) void k_setup(int nodes, const int* __restrict_
     minv[v] = kNoKey;
     const int beg = nindex[v], end = nindex[v + 1];
     if (end > beg) {
-      const long long g0 = ((long long)beg + task_edges - 1) / task_edges;
-      const long long g1 = ((long long)end + task_edges - 1) / task_edges;
+      const long long g0 = ((long long)beg + kStartEdges - 1) / kStartEdges;
+      const long long g1 = ((long long)end + kStartEdges - 1) / kStartEdges;
       for (long long g = g0; g < g1; g++) gsrc[g] = (int)v;
     }
   }
@@ -270,14 +273,15 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
   int* marks = reinterpret_cast<int*>(lds_stage + kInitWaves * stage_cap) + wid * TE;
   const long long ngroups = ((long long)edges + TE - 1) / TE;
   const long long nwaves = (long long)gridDim.x * kInitWaves;
+  constexpr int R = TE / kStartEdges;  // start table entries per task
   // The stream loads of a wave's next task are issued before the current
   // task is processed (after the current task's boundary window, so waiting
   // for that window does not wait for them).
   long long g = (long long)blockIdx.x * kInitWaves + wid;
   int nl[SUB], w[SUB], base = 0, vend = 0;
   if (g < ngroups) {
-    base = gsrc[g];
-    vend = g + 1 < ngroups ? gsrc[g + 1] : nodes - 1;
+    base = gsrc[g * R];
+    vend = g + 1 < ngroups ? gsrc[(g + 1) * R] : nodes - 1;
 #pragma unroll
     for (int u = 0; u < SUB; u++) {
       const long long p = g * TE + u * kWave + lane;
@@ -292,8 +296,8 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
     const long long gn = g + nwaves;
     int nl_n[SUB], w_n[SUB], base_n = 0, vend_n = 0;
     if (gn < ngroups) {
-      base_n = gsrc[gn];
-      vend_n = gn + 1 < ngroups ? gsrc[gn + 1] : nodes - 1;
+      base_n = gsrc[gn * R];
+      vend_n = gn + 1 < ngroups ? gsrc[(gn + 1) * R] : nodes - 1;
 #pragma unroll
       for (int u = 0; u < SUB; u++) {
         const long long p = gn * TE + u * kWave + lane;
@@ -417,6 +421,7 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_heavy_sparse(
     }
   };
   auto load_task = [&](long long t, int (&nl)[SUB], int (&wt)[SUB], int& base, int& vend) {
+    static_assert(TE == kStartEdges, "sparse tasks are start-table blocks");
     base = gsrc[t];
     vend = t + 1 < ngroups ? gsrc[t + 1] : nodes - 1;
 #pragma unroll
@@ -484,7 +489,7 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_heavy_sparse(
 }
 
 // Skip flags of the sparse heavy initialisation: task g (positions
-// g*task_edges ...) is skipped when every vertex in [gsrc[g], gsrc[g+1]]
+// g*kStartEdges ...) is skipped when every vertex in [gsrc[g], gsrc[g+1]]
 // (a superset of its sources) lies in the marked component (bit 0 of its
 // class).  Ranges longer than 4096 vertices are not examined and not
 // skipped.
@@ -1086,11 +1091,9 @@ InitFn init_fn(bool heavy, int sub)
   return k_init_wl<false, 8>;
 }
 
-SparseFn sparse_fn(int sub)
+SparseFn sparse_fn()
 {
-  if (sub == 4) return k_init_heavy_sparse<4>;
-  if (sub == 16) return k_init_heavy_sparse<16>;
-  return k_init_heavy_sparse<8>;
+  return k_init_heavy_sparse<kSparseSub>;
 }
 
 SelFn sel_fn(bool first, int waves)
@@ -1349,8 +1352,8 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
   }
   {
     const int blocks = (int)(((long long)nodes + 255) / 256);
-    hipLaunchKernelGGL(k_setup, dim3(blocks), dim3(256), 0, s, nodes, d_nindex, wb.parent, wb.minv, wb.gsrc,
-                       task_edges, d_cnt, d_stats);
+    hipLaunchKernelGGL(k_setup, dim3(blocks), dim3(256), 0, s, nodes, d_nindex, wb.parent, wb.minv, wb.gsrc, d_cnt,
+                       d_stats);
     const long long z16 = ((long long)edges + 15) / 16;
     const int zb = (int)std::min<long long>(std::max<long long>((z16 + 255) / 256, 1), (long long)ctx->cus * 16);
     hipLaunchKernelGGL(k_zero, dim3(zb), dim3(256), 0, s, d_inMST, (long long)edges);
@@ -1379,7 +1382,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       hipLaunchKernelGGL(k_flatten, dim3(blocks), dim3(256), 0, s, nodes, wb.parent, d_cnt, &d_cnt[2], wb.vclass,
                          dthr);
       if (sparse && edges > 0) {
-        const long long ngroups = ((long long)edges + task_edges - 1) / task_edges;
+        const long long ngroups = ((long long)edges + kStartEdges - 1) / kStartEdges;
         hipLaunchKernelGGL(k_task_skip, dim3((unsigned)((ngroups + 255) / 256)), dim3(256), 0, s, ngroups, nodes,
                            wb.gsrc, wb.vclass, wb.skip, dthr, &d_cnt[3]);
       }
@@ -1390,8 +1393,9 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       // component's share of the adjacency (k_pick_big) lets one of them run
       // and the other return at once.
       if (phase == 1 && sparse) {
-        const SparseFn f = sparse_fn(sub);
-        hipLaunchKernelGGL(f, dim3(z.init_grid), dim3(kInitWaves * kWave), init_lds, s, d_nindex, d_nlist, d_eweight,
+        const SparseFn f = sparse_fn();
+        const size_t sparse_lds = init_lds_bytes(stage_cap, kSparseSub * kWave);
+        hipLaunchKernelGGL(f, dim3(z.init_grid), dim3(kInitWaves * kWave), sparse_lds, s, d_nindex, d_nlist, d_eweight,
                            wb.gsrc, nodes, edges, wb.parent, wb.vclass, wb.skip, rix->rev, rix->mirrored ? 1 : 0,
                            wl[0], z.seg_cap, wb.segcnt, &d_cnt[0], stage_cap, d_stats, dthr, &d_cnt[2]);
       }
@@ -1546,9 +1550,9 @@ int emst_create(int device, emst_ctx** out)
       for (int h = 0; h < 2 && ok; h++)
         ok = ok && hipFuncSetAttribute((const void*)init_fn(h == 1, sub), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        dyn) == hipSuccess;
-      ok = ok && hipFuncSetAttribute((const void*)sparse_fn(sub), hipFuncAttributeMaxDynamicSharedMemorySize, dyn) ==
-                     hipSuccess;
     }
+    ok = ok && hipFuncSetAttribute((const void*)sparse_fn(), hipFuncAttributeMaxDynamicSharedMemorySize, dyn) ==
+                   hipSuccess;
   }
   if (!ok) {
     emst_destroy(ctx);
