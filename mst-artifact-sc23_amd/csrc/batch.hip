@@ -385,6 +385,7 @@ int emst_batch_solve(emst_batch* b, int use_filter, const int32_t* thresholds, i
       const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
       if (ei != hipSuccess) return ctx_fail(ctx, EMST_E_HIP, "hipGraphInstantiate failed");
+      if (c.exec) (void)hipGraphExecDestroy(c.exec);  // a capture of another mode
       c.exec = exec;
       c.cap_mode = mode_key;
       c.cap_gen = ctx->buf_gen;

@@ -105,3 +105,39 @@ def test_batch_graph_replays(setup):
         b.solve(emst.FILTER_OFF, use_graph=False)
         check_all(b, graphs, refs)
     b.close()
+
+
+def test_batch_stale_capture_falls_back(setup):
+    """A capture holds the round counts of the solve it was taken from.  A
+    replay whose thresholds need more rounds than were captured leaves a
+    phase unfinished; its posted count is then non-zero, and the arena is
+    solved again dynamically and recaptured.  Captured with every edge light
+    (threshold 2^30), replayed with every edge heavy (threshold 0)."""
+    s, graphs, refs = setup
+    b = emst.Batch(s, graphs, max_arena_edges=50000)
+    all_light = np.full(len(graphs), 1 << 30, np.int32)
+    all_heavy = np.zeros(len(graphs), np.int32)
+    st = b.solve(emst.FILTER_PLAIN, thresholds=all_light, use_graph=True)
+    assert st.replays == 0 and st.heavy == 0
+    check_all(b, graphs, refs)
+    st = b.solve(emst.FILTER_PLAIN, thresholds=all_heavy, use_graph=True)
+    assert st.replays < st.chunks      # at least one arena's capture was stale
+    check_all(b, graphs, refs)
+    st = b.solve(emst.FILTER_PLAIN, thresholds=all_heavy, use_graph=True)
+    assert st.replays == st.chunks     # recaptured: every arena replays
+    check_all(b, graphs, refs)
+    b.close()
+
+
+def test_batch_learned_graph_replay(setup):
+    """Learned per-graph thresholds with capture, then a replay."""
+    s, graphs, refs = setup
+    b = emst.Batch(s, graphs, max_arena_edges=50000)
+    b.set_filter_model(np.fromfile(os.path.join(ROOT, "mst-artifact-sc23_amd", "filter_mlp_union.f32"), np.float32))
+    st = b.solve(emst.FILTER_LEARNED, use_graph=True)
+    assert st.replays == 0
+    check_all(b, graphs, refs)
+    st = b.solve(emst.FILTER_LEARNED, use_graph=True)
+    assert st.replays == st.chunks
+    check_all(b, graphs, refs)
+    b.close()

@@ -134,3 +134,41 @@ def test_c5_rmat26_ties_properties():
     print(f"  OpenMP Boruvka {time.time()-t1:.1f}s", flush=True)
     assert emst.mst_weight(g, b) == w
     assert np.array_equal(b, base)
+
+
+def test_c4_batch_4096_every_graph():
+    """C4 at its real size: the 4,096-graph batch of the bench (seed 4, n
+    log-uniform in [1e3, 1e6], grid / geometric / RMAT ef 8 / ER deg 10,
+    weights in [1, 2^20)), about 5.3e9 directed edges in three arenas of
+    just under 2^31 edges (64-bit arena offsets).  Plain per-graph rule and
+    learned thresholds (union network), each solved with capture and then
+    replayed from the captured hipGraph; after every solve every graph's
+    inMST must equal the oracle's Kruskal of that graph alone."""
+    import sys
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, ROOT)
+    import bench
+    t0 = time.time()
+    graphs = [g for _, g in bench.batch_graphs(emst, 4096, 4, 1000, 1_000_000)]
+    tot = sum(g.edges for g in graphs)
+    print(f"c4: {len(graphs)} graphs, {tot} edges, generated in {time.time()-t0:.0f}s", flush=True)
+    t0 = time.time()
+    with ThreadPoolExecutor(16) as ex:   # the oracle's ctypes calls run in parallel
+        refs = list(ex.map(O.kruskal, graphs))
+    print(f"  oracle Kruskal of every graph in {time.time()-t0:.0f}s", flush=True)
+    s = _solver()
+    b = emst.Batch(s, graphs)
+    b.set_filter_model(np.fromfile(os.path.join(ROOT, "mst-artifact-sc23_amd", "filter_mlp_union.f32"), np.float32))
+    chunks, nodes, edges = b.info()
+    assert edges == tot and chunks == 3
+    for mode in (emst.FILTER_PLAIN, emst.FILTER_LEARNED):
+        for rep in range(2):
+            st = b.solve(mode, use_graph=True)
+            print(f"  mode {mode} solve {rep}: {st.ms:.1f} ms, arenas {st.chunks}, replays {st.replays}, "
+                  f"filtered {st.filtered_graphs}", flush=True)
+            assert st.graphs == 4096 and st.chunks == 3
+            assert st.replays == (3 if rep else 0)   # the second solve replays every arena's capture
+            bad = [i for i in range(len(graphs)) if not np.array_equal(b.fetch(i), refs[i])]
+            assert not bad, bad[:10]
+    b.close()
+    s.close()
