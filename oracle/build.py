@@ -1,10 +1,17 @@
 """Builds the oracle (test infrastructure only) and, where the reference
-tree is present, the reference-side checker in oracle/_ref/.
+tree is present, the reference-side checkers in oracle/_ref/.
 
   oracle/_build/liboracle.so   gcc build of emst_oracle.c + cpu_baselines.c
   oracle/_ref/ref_egr_dump     g++ build of oracle/ref_egr_dump.cpp against the
                                reference's own ECLgraph.h (read in place from
                                /root/reference/ECL-MST, never copied)
+  oracle/_ref/ecl2pbbs, ecl2lsg, ecl2mtx, ecl2uminho
+                               the reference's format converters
+                               (*/inputs/ecl2*_genweights.cpp), compiled in
+                               place with each directory's own ECLgraph.h, as
+                               the reference's build scripts do
+                               (PBBS_CPU/build_PBBS.sh:15 and the others, minus
+                               -march=native)
 
 The rest of the reference's hot path (ECL_MST.cu) needs nvcc and <cuda.h>,
 neither of which exists here, so it is not built.
@@ -15,6 +22,14 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_ECL = "/root/reference/ECL-MST"
+REF_ROOT = "/root/reference"
+# converter name -> (directory under the reference tree, source file)
+REF_CONVERTERS = {
+    "ecl2pbbs": ("PBBS_CPU/inputs", "ecl2pbbs_genweights.cpp"),
+    "ecl2lsg": ("Lonestar_CPU/inputs", "ecl2lsg_genweights.cpp"),
+    "ecl2mtx": ("cuGraph_GPU/inputs", "ecl2mtx_genweights_cugraph.cpp"),
+    "ecl2uminho": ("UMinho_BOTH/inputs", "ecl2UMinho_genweights.cpp"),
+}
 
 
 def _run(cmd):
@@ -40,6 +55,12 @@ def build():
         exe = os.path.join(refdir, "ref_egr_dump")
         _run(["g++", "-O2", "-I", REF_ECL, "-o", exe, os.path.join(HERE, "ref_egr_dump.cpp")])
         built.append(exe)
+        for name, (d, src) in REF_CONVERTERS.items():
+            path = os.path.join(REF_ROOT, d, src)
+            if os.path.exists(path):
+                exe = os.path.join(refdir, name)
+                _run(["g++", "-O3", "-w", "-I", os.path.join(REF_ROOT, d), "-o", exe, path])
+                built.append(exe)
     return built
 
 

@@ -372,6 +372,41 @@ def test_format_writers(tmp_path):
         emst.write_format(str(tmp_path / "x.mtx"), emst.Graph(g.nindex, g.nlist, None), "mtx")
 
 
+REF_BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref")
+REF_CONVERTERS = {"mtx": "ecl2mtx", "pbbs": "ecl2pbbs", "galois": "ecl2lsg", "uminho": "ecl2uminho"}
+
+
+@pytest.mark.skipif(not all(os.path.exists(os.path.join(REF_BIN, c)) for c in REF_CONVERTERS.values()),
+                    reason="reference converters not built (needs /root/reference)")
+def test_format_writers_match_reference_converters(tmp_path):
+    """Each baseline-format writer against the reference's own converter
+    (PBBS_CPU/inputs/ecl2pbbs_genweights.cpp, Lonestar_CPU/inputs/
+    ecl2lsg_genweights.cpp, cuGraph_GPU/inputs/ecl2mtx_genweights_cugraph.cpp,
+    UMinho_BOTH/inputs/ecl2UMinho_genweights.cpp), compiled from its source
+    into oracle/_ref, on the same .egr bytes: the output files must be
+    identical byte for byte.  Inputs: a one-way multigraph with an odd edge
+    count (the .lsg pad word), an RMAT graph, weights above 2^24 and negative
+    ones (float rounding and sign in the text formats), and a weightless file
+    (both sides synthesize the weights of ECL_MST.cu:410-419)."""
+    rng = np.random.default_rng(41)
+    rm = emst.gen_rmat(10, 8, 0.57, 0.19, 0.19, 3, 1, 1 << 24)
+    big = emst.Graph(rm.nindex, rm.nlist, rng.integers(1 << 24, (1 << 31) - 1, rm.edges).astype(np.int32))
+    neg = emst.Graph(rm.nindex, rm.nlist, rng.integers(-(1 << 20), 1 << 20, rm.edges).astype(np.int32))
+    grid = emst.gen_grid(30, 17, 2)
+    cases = {"oneway": _fmt_graph(), "rmat": rm, "big": big, "neg": neg,
+             "weightless": emst.Graph(grid.nindex, grid.nlist, None)}
+    for name, g in cases.items():
+        egr = tmp_path / f"{name}.egr"
+        emst.write_egr(str(egr), g)
+        ours_g = emst.read_egr(str(egr), synth_weights=True)
+        for fmt, conv in REF_CONVERTERS.items():
+            ref_out, our_out = tmp_path / f"{name}.{fmt}.ref", tmp_path / f"{name}.{fmt}.ours"
+            r = subprocess.run([os.path.join(REF_BIN, conv), str(egr), str(ref_out)], capture_output=True, text=True)
+            assert r.returncode == 0, (name, fmt, r.stderr)
+            emst.write_format(str(our_out), ours_g, fmt)
+            assert our_out.read_bytes() == ref_out.read_bytes(), (name, fmt)
+
+
 def test_gen_and_convert_tools(tmp_path):
     """bin/ecl_gen writes the generators' graphs; bin/ecl_convert adds the
     reference's synthetic weights to a weightless input and converts."""
