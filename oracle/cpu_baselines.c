@@ -38,56 +38,81 @@ static inline uint64_t okey(int32_t w, int32_t j)
 /* ------------------------------------------------------------------------ */
 /* Contraction Boruvka                                                      */
 /* ------------------------------------------------------------------------ */
+
+/* In-place exclusive prefix sum of a[0..n) in parallel (per-thread block
+ * sums, then their offsets); returns the total. */
+static int64_t par_exscan(int64_t* a, int64_t n)
+{
+  int64_t total = 0;
+  const int nt = omp_get_max_threads();
+  int64_t* part = (int64_t*)calloc((size_t)nt + 1, sizeof(int64_t));
+#pragma omp parallel num_threads(nt)
+  {
+    const int t = omp_get_thread_num(), T = omp_get_num_threads();
+    const int64_t b = n * t / T, e = n * (t + 1) / T;
+    int64_t acc = 0;
+    for (int64_t i = b; i < e; i++) acc += a[i];
+    part[t + 1] = acc;
+#pragma omp barrier
+#pragma omp single
+    for (int i = 0; i < T; i++) part[i + 1] += part[i];
+    acc = part[t];
+    for (int64_t i = b; i < e; i++) {
+      const int64_t x = a[i];
+      a[i] = acc;
+      acc += x;
+    }
+  }
+  total = part[nt];
+  free(part);
+  return total;
+}
+
 int orc_boruvka_omp(const orc_graph* g, uint8_t* inMST, int threads)
 {
   if (threads > 0) omp_set_num_threads(threads);
   const int32_t n0 = g->nodes;
   memset(inMST, 0, (size_t)g->edges);
-  /* current graph: CSR over super-vertices; each entry (dst, key) */
-  int64_t m_up = 0;
-  for (int32_t v = 0; v < n0; v++)
-    for (int32_t j = g->nindex[v]; j < g->nindex[v + 1]; j++)
-      if (g->nlist[j] != v) m_up++;
-  int64_t* idx = (int64_t*)malloc(((size_t)n0 + 1) * sizeof(int64_t));
-  int32_t* dst = (int32_t*)malloc((size_t)(m_up ? m_up : 1) * sizeof(int32_t));
-  uint64_t* key = (uint64_t*)malloc((size_t)(m_up ? m_up : 1) * sizeof(uint64_t));
-  if (!idx || !dst || !key) { free(idx); free(dst); free(key); return ORC_E_ALLOC; }
-  /* Both directions of every non-loop edge, keyed by its upper position j.
+  /* The current graph is a CSR over super-vertices, each entry (dst, key).
    * The reference's MST only considers upper-direction entries (n > v); a
    * lower entry v->u (u < v) stands for the same undirected edge only on a
    * symmetric input, so here each upper entry j contributes (v->u, j) and
-   * (u->v, j). */
-  int64_t* deg = (int64_t*)calloc((size_t)n0 + 1, sizeof(int64_t));
+   * (u->v, j).  Entries are placed with atomic cursors: the order inside a
+   * list is not fixed, and need not be, since every key is unique. */
+  int64_t* idx = (int64_t*)calloc((size_t)n0 + 1, sizeof(int64_t));
+  if (!idx) return ORC_E_ALLOC;
+#pragma omp parallel for schedule(dynamic, 4096)
   for (int32_t v = 0; v < n0; v++)
     for (int32_t j = g->nindex[v]; j < g->nindex[v + 1]; j++) {
       const int32_t u = g->nlist[j];
-      if (u > v) { deg[v]++; deg[u]++; }
+      if (u > v) {
+        __atomic_fetch_add(&idx[v], 1, __ATOMIC_RELAXED);
+        __atomic_fetch_add(&idx[u], 1, __ATOMIC_RELAXED);
+      }
     }
-  idx[0] = 0;
-  for (int32_t v = 0; v < n0; v++) idx[v + 1] = idx[v] + deg[v];
-  const int64_t m = idx[n0];
-  if (m > m_up) {  /* reallocate for the doubled list */
-    dst = (int32_t*)realloc(dst, (size_t)(m ? m : 1) * sizeof(int32_t));
-    key = (uint64_t*)realloc(key, (size_t)(m ? m : 1) * sizeof(uint64_t));
-  }
+  const int64_t m = par_exscan(idx, (int64_t)n0 + 1);
+  int32_t* dst = (int32_t*)malloc((size_t)(m ? m : 1) * sizeof(int32_t));
+  uint64_t* key = (uint64_t*)malloc((size_t)(m ? m : 1) * sizeof(uint64_t));
   int64_t* cur = (int64_t*)malloc(((size_t)n0 + 1) * sizeof(int64_t));
+  if (!dst || !key || !cur) { free(idx); free(dst); free(key); free(cur); return ORC_E_ALLOC; }
   memcpy(cur, idx, ((size_t)n0 + 1) * sizeof(int64_t));
+#pragma omp parallel for schedule(dynamic, 4096)
   for (int32_t v = 0; v < n0; v++)
     for (int32_t j = g->nindex[v]; j < g->nindex[v + 1]; j++) {
       const int32_t u = g->nlist[j];
       if (u > v) {
         const uint64_t k = okey(g->eweight[j], j);
-        dst[cur[v]] = u; key[cur[v]++] = k;
-        dst[cur[u]] = v; key[cur[u]++] = k;
+        const int64_t a = __atomic_fetch_add(&cur[v], 1, __ATOMIC_RELAXED);
+        const int64_t b = __atomic_fetch_add(&cur[u], 1, __ATOMIC_RELAXED);
+        dst[a] = u; key[a] = k;
+        dst[b] = v; key[b] = k;
       }
     }
-  free(cur);
-  free(deg);
   int32_t n = n0;
-  int32_t* succ = (int32_t*)malloc((size_t)n0 * sizeof(int32_t));
-  int32_t* color = (int32_t*)malloc((size_t)n0 * sizeof(int32_t));
-  int32_t* newid = (int32_t*)malloc((size_t)n0 * sizeof(int32_t));
-  uint64_t* minkey = (uint64_t*)malloc((size_t)n0 * sizeof(uint64_t));
+  int32_t* succ = (int32_t*)malloc((size_t)(n0 ? n0 : 1) * sizeof(int32_t));
+  int32_t* color = (int32_t*)malloc((size_t)(n0 ? n0 : 1) * sizeof(int32_t));
+  int64_t* newid = (int64_t*)malloc(((size_t)n0 + 1) * sizeof(int64_t));
+  uint64_t* minkey = (uint64_t*)malloc((size_t)(n0 ? n0 : 1) * sizeof(uint64_t));
   int64_t* cnt = (int64_t*)malloc(((size_t)n0 + 1) * sizeof(int64_t));
   while (n > 1 && idx[n] > 0) {
     /* find_min_per_vertex */
@@ -122,30 +147,37 @@ int orc_boruvka_omp(const orc_graph* g, uint8_t* inMST, int threads)
         if (c != color[u]) { color[u] = c; changed = 1; }
       }
     }
-    /* new ids for the roots */
-    int32_t nn = 0;
-    for (int32_t u = 0; u < n; u++)
-      if (color[u] == u) newid[u] = nn++;
-    /* count surviving edges per new super-vertex */
-    memset(cnt, 0, ((size_t)nn + 1) * sizeof(int64_t));
+    /* new ids for the roots: exclusive scan of the root flags */
+#pragma omp parallel for
+    for (int32_t u = 0; u < n; u++) newid[u] = color[u] == u;
+    const int32_t nn = (int32_t)par_exscan(newid, n);
+    /* count the surviving edges of every new super-vertex */
+#pragma omp parallel for
+    for (int32_t v = 0; v <= nn; v++) cnt[v] = 0;
+#pragma omp parallel for schedule(dynamic, 1024)
     for (int32_t u = 0; u < n; u++) {
       const int32_t cu = color[u];
       int64_t c = 0;
       for (int64_t e = idx[u]; e < idx[u + 1]; e++)
         if (color[dst[e]] != cu) c++;
-      cnt[newid[cu] + 1] += c;
+      if (c) __atomic_fetch_add(&cnt[newid[cu]], c, __ATOMIC_RELAXED);
     }
-    for (int32_t v = 0; v < nn; v++) cnt[v + 1] += cnt[v];
-    const int64_t mm = cnt[nn];
+    const int64_t mm = par_exscan(cnt, (int64_t)nn + 1);
     int32_t* ndst = (int32_t*)malloc((size_t)(mm ? mm : 1) * sizeof(int32_t));
     uint64_t* nkey = (uint64_t*)malloc((size_t)(mm ? mm : 1) * sizeof(uint64_t));
     int64_t* pos = (int64_t*)malloc(((size_t)nn + 1) * sizeof(int64_t));
     memcpy(pos, cnt, ((size_t)nn + 1) * sizeof(int64_t));
+    /* copy: each vertex reserves a block of its super-vertex's list */
+#pragma omp parallel for schedule(dynamic, 1024)
     for (int32_t u = 0; u < n; u++) {
-      const int32_t cu = newid[color[u]];
+      const int32_t cu = color[u];
+      int64_t c = 0;
+      for (int64_t e = idx[u]; e < idx[u + 1]; e++) c += color[dst[e]] != cu;
+      if (!c) continue;
+      int64_t at = __atomic_fetch_add(&pos[newid[cu]], c, __ATOMIC_RELAXED);
       for (int64_t e = idx[u]; e < idx[u + 1]; e++) {
         const int32_t cv = color[dst[e]];
-        if (cv != color[u]) { ndst[pos[cu]] = newid[cv]; nkey[pos[cu]++] = key[e]; }
+        if (cv != cu) { ndst[at] = (int32_t)newid[cv]; nkey[at++] = key[e]; }
       }
     }
     free(pos);
@@ -154,7 +186,7 @@ int orc_boruvka_omp(const orc_graph* g, uint8_t* inMST, int threads)
     memcpy(idx, cnt, ((size_t)nn + 1) * sizeof(int64_t));
     n = nn;
   }
-  free(idx); free(dst); free(key); free(succ); free(color); free(newid); free(minkey); free(cnt);
+  free(idx); free(dst); free(key); free(succ); free(color); free(newid); free(minkey); free(cnt); free(cur);
   return 0;
 }
 
