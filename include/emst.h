@@ -268,6 +268,13 @@ int emst_batch_solve(emst_batch* b, int use_filter, const int32_t* thresholds, i
 int emst_batch_fetch(emst_batch* b, int32_t g, uint8_t* inMST);
 /* Per-graph thresholds and filter flags of the last solve. */
 int emst_batch_thresholds(const emst_batch* b, int32_t* thresholds, int32_t* use_filter);
+/* Device time of each arena in the last solve (arenas run one after another
+ * on the context's stream, and every graph of an arena completes with it),
+ * and the index of each arena's first graph (arena i holds graphs
+ * first_graph[i] .. first_graph[i+1]-1): up to cap entries each, the arena
+ * count in *arenas.  No counterpart in the reference, which solves one graph
+ * per run. */
+int emst_batch_arena_ms(const emst_batch* b, double* ms, int32_t* first_graph, int32_t cap, int32_t* arenas);
 /* Network used by EMST_FILTER_LEARNED for this batch (EMST_MLP_PARAMS fp32
  * values, as emst_set_filter_model); NULL falls back to the context's.  The
  * rounds of an arena are shared by all its graphs, so a graph's cost there

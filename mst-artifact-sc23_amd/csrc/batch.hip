@@ -63,6 +63,7 @@ struct emst_batch {
   std::vector<int32_t> graph_chunk;        // chunk of each graph
   std::vector<int32_t> plain_thr, plain_flag;  // plain rule per graph (host)
   std::vector<int32_t> last_thr, last_flag;
+  std::vector<double> last_arena_ms;  // device time of each arena in the last solve
   long long total_nodes = 0, total_edges = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   unsigned short* d_mlp16 = nullptr;  // the batch's own network (bf16), if set
@@ -272,6 +273,7 @@ int emst_batch_solve(emst_batch* b, int use_filter, const int32_t* thresholds, i
   hipStream_t s = ctx->stream;
   if (st) std::memset(st, 0, sizeof(*st));
   double total_ms = 0;
+  b->last_arena_ms.assign(b->chunks.size(), 0.0);
   for (size_t ci = 0; ci < b->chunks.size(); ci++) {
     Chunk& c = b->chunks[ci];
     const int32_t G = c.g1 - c.g0;
@@ -343,6 +345,7 @@ int emst_batch_solve(emst_batch* b, int use_filter, const int32_t* thresholds, i
         cs.heavy_rounds = (int32_t)hs[ST_ROUNDS1];
         cs.tail_rounds = (int32_t)hs[ST_TAIL];
         total_ms += ms;
+        b->last_arena_ms[ci] = ms;
         if (st) {
           st->light += cs.light;
           st->heavy += cs.heavy;
@@ -360,6 +363,7 @@ int emst_batch_solve(emst_batch* b, int use_filter, const int32_t* thresholds, i
                        any ? EMST_FILTER_PLAIN : EMST_FILTER_OFF, INT_MAX, c.d_inMST, s, c.m_up, &vt, &cs);
     if (rc) return rc;
     total_ms += cs.ms;
+    b->last_arena_ms[ci] = cs.ms;
     if (st) {
       st->light += cs.light;
       st->heavy += cs.heavy;
@@ -426,6 +430,18 @@ int emst_batch_thresholds(const emst_batch* b, int32_t* thresholds, int32_t* use
   for (int32_t g = 0; g < b->ngraphs; g++) {
     if (thresholds) thresholds[g] = b->last_thr[g];
     if (use_filter) use_filter[g] = b->last_flag[g];
+  }
+  return EMST_OK;
+}
+
+int emst_batch_arena_ms(const emst_batch* b, double* ms, int32_t* first_graph, int32_t cap, int32_t* arenas)
+{
+  if (!b || (cap > 0 && (!ms || !first_graph))) return EMST_E_ARG;
+  const int32_t n = (int32_t)b->chunks.size();
+  if (arenas) *arenas = n;
+  for (int32_t i = 0; i < n && i < cap; i++) {
+    ms[i] = i < (int32_t)b->last_arena_ms.size() ? b->last_arena_ms[i] : 0.0;
+    first_graph[i] = b->chunks[i].g0;
   }
   return EMST_OK;
 }

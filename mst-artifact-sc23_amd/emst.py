@@ -175,6 +175,8 @@ _batch_solve = _sig("emst_batch_solve", C.c_int, C.c_void_p, C.c_int, _p32, C.c_
 _batch_fetch = _sig("emst_batch_fetch", C.c_int, C.c_void_p, C.c_int32, _pu8)
 _batch_thresholds = _sig("emst_batch_thresholds", C.c_int, C.c_void_p, _p32, _p32)
 _batch_set_model = _sig("emst_batch_set_filter_model", C.c_int, C.c_void_p, _pf32)
+_batch_arena_ms = _sig("emst_batch_arena_ms", C.c_int, C.c_void_p, C.POINTER(C.c_double), _p32, C.c_int32,
+                       C.POINTER(C.c_int32))
 _batch_info = _sig("emst_batch_info", C.c_int, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                    C.POINTER(C.c_int64))
 class ContractStats(C.Structure):
@@ -575,6 +577,16 @@ class Batch:
         assert p.shape == (MLP_PARAMS,)
         self._model = p
         self.solver._err(_batch_set_model(self._h, p.ctypes.data_as(_pf32)))
+
+    def arena_ms(self):
+        """(device ms of each arena in the last solve, index of each arena's first graph)."""
+        n = C.c_int32()
+        self.solver._err(_batch_arena_ms(self._h, None, None, 0, C.byref(n)))
+        ms = np.zeros(max(n.value, 1), np.float64)
+        first = np.zeros(max(n.value, 1), np.int32)
+        self.solver._err(_batch_arena_ms(self._h, ms.ctypes.data_as(C.POINTER(C.c_double)),
+                                         first.ctypes.data_as(_p32), n.value, C.byref(n)))
+        return ms[:n.value], first[:n.value]
 
     def thresholds(self):
         t = np.zeros(len(self.graphs), np.int32)
