@@ -186,7 +186,12 @@ int emst_gpu_mst(emst_ctx* ctx, const emst_graph* g, int use_filter, int32_t thr
  * emst_load does the allocations and host-to-device copies of lines
  * 268-297 (validating the CSR first), emst_solve is the device-resident
  * timed region, emst_fetch is the device-to-host copy of line 347.  A
- * loaded graph stays resident across any number of solves. */
+ * loaded graph stays resident across any number of solves.  With
+ * EMST_FILTER_LEARNED, emst_solve decides the threshold of a loaded graph
+ * once (the network's inference on the device), before its timed region,
+ * and reuses it until the next emst_load or emst_set_filter_model, as the
+ * reference decides its threshold before calling gpuMST (ECL_MST.cu:432-441);
+ * emst_solve_device decides it inside every solve. */
 int emst_load(emst_ctx* ctx, const emst_graph* g);
 int emst_solve(emst_ctx* ctx, int use_filter, int32_t threshold, emst_stats* stats);
 int emst_fetch(emst_ctx* ctx, uint8_t* inMST);

@@ -71,6 +71,11 @@ struct emst_ctx {
   float* d_q = nullptr;
   float* d_feat = nullptr;
   int* d_samples = nullptr;
+  // the learned threshold of the loaded graph, decided once outside the
+  // timed region (as the reference decides its threshold before gpuMST,
+  // ECL_MST.cu:432-441); d_thr and d_q hold it while learned_cached is set
+  bool learned_cached = false;
+  int learned_filter = 0;
   // incremented whenever a work buffer is (re)allocated: captured launch
   // schedules hold raw buffer addresses and are valid for one generation
   unsigned long long buf_gen = 0;
@@ -143,7 +148,7 @@ int ensure_buffers(emst_ctx* ctx, int32_t nodes, int32_t edges, long long wl_cap
 int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
               const int* d_eweight, int use_filter, int32_t threshold, uint8_t* d_inMST, hipStream_t s,
               long long wl_cap, const VertexThresholds* vt, emst_stats* stats, FixedSchedule* fixed = nullptr,
-              const WorkBufs* wb = nullptr, const RevIndex* rix = nullptr);
+              const WorkBufs* wb = nullptr, const RevIndex* rix = nullptr, bool learned_cached = false);
 
 // Builds the reverse positions of a device-resident graph into rev (edges
 // ints) on stream s; *ok is set when the lists are sorted and every

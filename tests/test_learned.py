@@ -152,3 +152,57 @@ def test_trained_model_beats_plain_rule(which):
     gm = math.exp(np.mean(np.log(ratios)))
     print(which, "geomean learned/plain modelled cost", gm)
     assert gm < 0.95
+
+
+@gpu
+def test_learned_threshold_decided_once_per_loaded_graph():
+    """emst_solve decides a loaded graph's learned threshold once, before its
+    timed region, and reuses it until the next load or model change; the
+    device-pointer entry decides it inside the solve.  Both give the same
+    threshold, and a model change or a new graph is seen at once."""
+    import ctypes as C
+    s = emst.Solver(0)
+    s.set_filter_model(np.fromfile(MODEL, np.float32))
+    g1, g2 = emst.gen_rmat(14, 16, seed=3), emst.gen_rgg(30000, 6.0, 4)
+    s.load(g1)
+    a = s.solve(emst.FILTER_LEARNED, 0)
+    got_a = s.fetch()
+    b = s.solve(emst.FILTER_LEARNED, 0)
+    assert (a.threshold_used, a.filter_used, a.learned_q) == (b.threshold_used, b.filter_used, b.learned_q)
+    assert np.array_equal(s.fetch(), got_a) and np.array_equal(got_a, O.kruskal(g1))
+    uf, T, q, _, _ = s.learned_threshold()
+    assert (T, int(uf)) == (a.threshold_used, a.filter_used)
+    # the device-pointer entry on the same graph decides inside the solve
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    bufs = []
+
+    def dev(arr):
+        p = C.c_void_p()
+        assert hip.hipMalloc(C.byref(p), max(arr.nbytes, 1)) == 0
+        bufs.append(p)
+        assert hip.hipMemcpy(p, arr.ctypes.data, arr.nbytes, 1) == 0
+        return p.value
+    try:
+        out = np.zeros(g1.edges, np.uint8)
+        d_out = dev(out)
+        c = s.solve_device(g1.nodes, g1.edges, dev(g1.nindex), dev(g1.nlist), dev(g1.eweight), emst.FILTER_LEARNED,
+                           0, d_out)
+        assert hip.hipMemcpy(out.ctypes.data, C.c_void_p(d_out), g1.edges, 2) == 0
+    finally:
+        for p in bufs:
+            hip.hipFree(p)
+    assert (c.threshold_used, c.filter_used) == (a.threshold_used, a.filter_used)
+    assert np.array_equal(out, got_a)
+    # a new graph, then no model (the plain rule): both seen by the next solve
+    s.load(g2)
+    d = s.solve(emst.FILTER_LEARNED, 0)
+    uf2, T2, _, _, _ = s.learned_threshold()
+    assert (d.threshold_used, d.filter_used) == (T2, int(uf2))
+    s.set_filter_model(None)
+    e = s.solve(emst.FILTER_LEARNED, 0)
+    assert (bool(e.filter_used), e.threshold_used) == emst.plain_threshold(g2) and e.learned_q < 0
+    assert np.array_equal(s.fetch(), O.kruskal(g2))
+    s.close()
