@@ -40,6 +40,8 @@ struct emst_ctx {
   size_t cap_seg = 0;
   unsigned char* d_skip = nullptr;    // per-task skip flags of the sparse heavy initialisation
   size_t cap_skip = 0;
+  unsigned char* d_hasmem = nullptr;  // per-vertex flag: a root with other members (sparse heavy initialisation)
+  size_t cap_hasmem = 0;
   // reverse positions of the loaded graph (emst_load), when its lists are
   // sorted and symmetric
   int* d_rev = nullptr;
@@ -96,6 +98,7 @@ struct WorkBufs {
   long long* stats = nullptr;   // ST_COUNT counters
   int* segcnt = nullptr;        // records per initialisation segment
   unsigned char* skip = nullptr;  // per-task skip flags (sparse heavy initialisation)
+  unsigned char* hasmem = nullptr;  // nodes: root-with-members flags (sparse heavy initialisation)
 };
 
 // Reverse positions of a graph whose lists are sorted and symmetric: rev[p]

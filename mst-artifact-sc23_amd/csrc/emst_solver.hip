@@ -72,7 +72,7 @@ size_t init_lds_bytes(int stage_recs, int task_edges)
 __global__ __launch_bounds__(256) void k_setup(int nodes, const int* __restrict__ nindex,
                                                int* __restrict__ parent, u64* __restrict__ minv,
                                                int* __restrict__ gsrc, int* __restrict__ cnt,
-                                               long long* __restrict__ stats)
+                                               long long* __restrict__ stats, unsigned char* __restrict__ hasmem)
 {
   const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (v == 0) {
@@ -85,6 +85,7 @@ __global__ __launch_bounds__(256) void k_setup(int nodes, const int* __restrict_
   if (v < nodes) {
     parent[v] = (int)v;
     minv[v] = kNoKey;
+    if (hasmem) hasmem[v] = 0;
     const int beg = nindex[v], end = nindex[v + 1];
     if (end > beg) {
       const long long g0 = ((long long)beg + kStartEdges - 1) / kStartEdges;
@@ -360,39 +361,72 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
 }
 
 // ---------------------------------------------------------------------------
-// Sparse heavy initialisation (same output as k_init_wl<true>, for sorted,
-// symmetric lists with reverse positions).  A crossing edge has at least
-// one endpoint outside the marked component M, so only the positions of
-// sources outside M are examined, and a task whose source vertices all lie
-// in M is skipped without streaming it (skip[g], from k_task_skip).  For a
-// source v outside M and a position p of v with neighbour y:
+// Sparse heavy initialisation (the records of k_init_wl<true>, for sorted,
+// symmetric lists with reverse positions), fused with the heavy phase's
+// first round of offers.  A crossing edge has at least one endpoint outside
+// the marked component M, so only the positions of sources outside M are
+// examined, and a task whose source vertices all lie in M is skipped without
+// streaming it (skip[g], from k_task_skip).  For a source v outside M and a
+// position p of v with neighbour y:
 //   y > v (p is the upper copy): emitted when root(v) != root(y), as in
 //       k_init_wl<true>;
 //   y < v, y in M (p is the lower copy of an edge crossing out of M): the
 //       edge's upper copy is rev[p] in y's list, so the record is
 //       {root(y), root(v), eweight[rev[p]], rev[p]};
-//   y < v, y outside M: skipped; y's own walk emits that edge's upper copy.
+//   y < v, y outside M: not emitted; y's own walk emits that edge's upper
+//       copy.
 // Every crossing upper-direction edge is thus emitted exactly once, the
 // reference's initWL<false> set (ECL_MST.cu:157-207, launch :326).  With
 // mirrored weights eweight[p] equals eweight[rev[p]] and is streamed with
-// the task.  The wave's tasks are g0, g0 + nwaves, ...; their skip flags are
-// read 64 at a time (one per lane) and only the tasks to do are visited,
-// the next one's stream loads issued before the current one is processed.
+// the task.
+// The first round's offers (kernel1's atomicMin, ECL_MST.cu:222-223) are
+// made here, so the offer-only select of that round is skipped: v's list
+// holds every edge incident to v (one copy of each, either direction), so
+// the minimum key over v's crossing positions, keyed by their upper copy, is
+// v's offer.  Per task it is reduced in LDS slots (one per source vertex);
+// the position holding the slot's minimum writes it to minv[v] with a plain
+// store when v is a component of its own (a root without members,
+// hasmem[v] == 0) whose list lies inside the task, else with a guarded
+// atomicMin to minv[root(v)].  M's offer is the minimum over the records
+// that touch M, reduced per wave and per workgroup, then one atomicMin per
+// workgroup.  A record carries candidate flags in bit 31 of x and y for the
+// first join: the source side is a candidate only if its key equals the
+// task's minimum for that source (it cannot be the component's minimum
+// otherwise); the other side is always a candidate.
+// The wave's tasks are g0, g0 + nwaves, ...; their skip flags are read 64 at
+// a time (one per lane) and only the tasks to do are visited, the next one's
+// stream loads issued before the current one is processed.
 // ---------------------------------------------------------------------------
+constexpr int kCandBit = INT_MIN;  // bit 31 of a record endpoint: candidate of the first join
+
+// Dynamic LDS of the sparse heavy initialisation: the initialisation layout,
+// then per-wave minimum slots for the sources of a task.
+size_t sparse_lds_bytes(int stage_recs)
+{
+  return init_lds_bytes(stage_recs, kSparseSub * kWave) + (size_t)kInitWaves * (kMarkSpan + 1) * sizeof(u64);
+}
+
 template <int SUB>
-__global__ __launch_bounds__(kInitWaves * kWave) void k_init_heavy_sparse(
+__global__ __launch_bounds__(kInitWaves * kWave) __attribute__((amdgpu_waves_per_eu(5))) void k_init_heavy_sparse(
     const int* __restrict__ nindex, const int* __restrict__ nlist, const int* __restrict__ eweight,
     const int* __restrict__ gsrc, int nodes, int edges, const int* __restrict__ label,
     const unsigned* __restrict__ vclass, const unsigned char* __restrict__ skip, const int* __restrict__ rev,
-    int mirrored, int4* __restrict__ wl, long long seg_cap, int* __restrict__ segcnt, int* __restrict__ cnt,
-    int stage_cap, long long* __restrict__ stats, const int* __restrict__ dthr, const int* __restrict__ bigroot_p)
+    int mirrored, const unsigned char* __restrict__ hasmem, u64* __restrict__ minv, int4* __restrict__ wl,
+    long long seg_cap, int* __restrict__ segcnt, int* __restrict__ cnt, int stage_cap, long long* __restrict__ stats,
+    const int* __restrict__ dthr, const int* __restrict__ bigroot_p)
 {
   constexpr int TE = SUB * kWave;
+  constexpr int NSLOT = kMarkSpan + 1;
   extern __shared__ int4 lds_stage[];
-  int& s_fill = *(reinterpret_cast<int*>(lds_stage + kInitWaves * stage_cap) + kInitWaves * TE);
+  int* const tail = reinterpret_cast<int*>(lds_stage + kInitWaves * stage_cap) + kInitWaves * TE;
+  int& s_fill = tail[0];
+  u64& s_bigmin = *reinterpret_cast<u64*>(tail + 2);
   if (dthr && dthr[1] == 0) return;  // no heavy phase when unfiltered
   if (!bigroot_p[1]) return;         // the full pass runs instead
-  if (threadIdx.x == 0) s_fill = 0;
+  if (threadIdx.x == 0) {
+    s_fill = 0;
+    s_bigmin = kNoKey;
+  }
   __syncthreads();
   int4* const seg = wl + (long long)blockIdx.x * seg_cap;
   const int bigroot = *bigroot_p;
@@ -400,6 +434,7 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_heavy_sparse(
   const int wid = threadIdx.x / kWave;
   Stage st{lds_stage + wid * stage_cap, stage_cap, 0};
   int* marks = reinterpret_cast<int*>(lds_stage + kInitWaves * stage_cap) + wid * TE;
+  u64* slot = reinterpret_cast<u64*>(tail + 4) + wid * NSLOT;
   const long long ngroups = ((long long)edges + TE - 1) / TE;
   const long long nwaves = (long long)gridDim.x * kInitWaves;
   const long long g0 = (long long)blockIdx.x * kInitWaves + wid;
@@ -420,7 +455,7 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_heavy_sparse(
       chunk++;
     }
   };
-  auto load_task = [&](long long t, int (&nl)[SUB], int (&wt)[SUB], int& base, int& vend) {
+  auto load_task = [&](long long t, int (&nl)[SUB], int& base, int& vend) {
     static_assert(TE == kStartEdges, "sparse tasks are start-table blocks");
     base = gsrc[t];
     vend = t + 1 < ngroups ? gsrc[t + 1] : nodes - 1;
@@ -428,64 +463,117 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_heavy_sparse(
     for (int u = 0; u < SUB; u++) {
       const long long p = t * TE + u * kWave + lane;
       nl[u] = p < edges ? ld_stream(&nlist[p]) : -1;
-      wt[u] = (mirrored && p < edges) ? ld_stream(&eweight[p]) : 0;
     }
   };
   long long g = next_task();
-  int nl[SUB], w[SUB], base = 0, vend = 0;
-  if (g >= 0) load_task(g, nl, w, base, vend);
+  int nl[SUB], base = 0, vend = 0;
+  if (g >= 0) load_task(g, nl, base, vend);
   while (g >= 0) {
     const long long gbeg = g * TE;
     TaskBounds tb;
     task_bounds_load(tb, nindex, nodes, base, vend, lane);
     const long long gn = next_task();
-    int nl_n[SUB], w_n[SUB], base_n = 0, vend_n = 0;
-    if (gn >= 0) load_task(gn, nl_n, w_n, base_n, vend_n);
+    int nl_n[SUB], base_n = 0, vend_n = 0;
+    if (gn >= 0) load_task(gn, nl_n, base_n, vend_n);
+    // mirrored weights are streamed with the task (used at the keys)
+    int w[SUB];
+#pragma unroll
+    for (int u = 0; u < SUB; u++) {
+      const long long p = gbeg + u * kWave + lane;
+      w[u] = (mirrored && p < edges) ? ld_stream(&eweight[p]) : 0;
+    }
     int src[SUB];
     task_sources<SUB>(src, tb, gbeg, base, edges, nodes, nindex, marks, lane);
-    // positions whose source lies outside M, and their neighbours' classes
-    unsigned cv[SUB], cu[SUB];
-    bool act[SUB];
-#pragma unroll
-    for (int u = 0; u < SUB; u++) {
-      const long long p = gbeg + u * kWave + lane;
-      cv[u] = (vclass[(unsigned)src[u] >> 4] >> ((src[u] & 15) * 2)) & 3u;
-      act[u] = p < edges && !(cv[u] & 1u) && nl[u] != src[u];
-      cu[u] = act[u] ? (vclass[(unsigned)nl[u] >> 4] >> ((nl[u] & 15) * 2)) & 3u : 0u;
-    }
+    const int pbeg = (int)gbeg;  // positions fit in int (edges < 2^31)
+    // roots of both endpoints of the positions whose source lies outside M;
+    // a position crosses when they differ
     int a[SUB], b[SUB];
-    bool keep[SUB];
+    bool cross[SUB], tom[SUB], lower[SUB], alone[SUB];
 #pragma unroll
     for (int u = 0; u < SUB; u++) {
-      const bool upper = nl[u] > src[u];
-      keep[u] = act[u] && (upper || (cu[u] & 1u));
-      a[u] = (cv[u] & 2u) ? src[u] : -1;
-      b[u] = (cu[u] & 1u) ? bigroot : (cu[u] & 2u) ? nl[u] : -1;
-      if (keep[u] && a[u] < 0) a[u] = label[src[u]];
-      if (keep[u] && b[u] < 0) b[u] = label[nl[u]];
-      keep[u] = keep[u] && a[u] != b[u];
+      const int p = pbeg + u * kWave + lane;
+      lower[u] = nl[u] < src[u];
+      const unsigned cv = (vclass[(unsigned)src[u] >> 4] >> ((src[u] & 15) * 2)) & 3u;
+      const bool act = p < edges && !(cv & 1u) && nl[u] != src[u];
+      const unsigned cu = act ? (vclass[(unsigned)nl[u] >> 4] >> ((nl[u] & 15) * 2)) & 3u : 0u;
+      tom[u] = cu & 1u;
+      a[u] = (cv & 2u) ? src[u] : -1;
+      b[u] = (cu & 1u) ? bigroot : (cu & 2u) ? nl[u] : -1;
+      // a root source without other members is a component of its own
+      alone[u] = act && a[u] >= 0 && !hasmem[src[u]];
+      if (act && a[u] < 0) a[u] = label[src[u]];
+      if (act && b[u] < 0) b[u] = label[nl[u]];
+      cross[u] = act && a[u] != b[u];
     }
+    // keys ((w, j) of the edge's upper-direction copy)
+    int j[SUB], wv[SUB];
 #pragma unroll
     for (int u = 0; u < SUB; u++) {
-      const long long p = gbeg + u * kWave + lane;
-      const bool lower = nl[u] < src[u];
-      int j = (int)p, wv = w[u];
-      if (keep[u] && lower) j = rev[p];
-      if (keep[u] && !mirrored) wv = eweight[j];
+      const int p = pbeg + u * kWave + lane;
+      j[u] = (cross[u] && lower[u]) ? rev[p] : p;
+      wv[u] = (cross[u] && !mirrored) ? eweight[j[u]] : w[u];
+    }
+    // each source's minimum over this task: LDS slots by vertex (vertex
+    // ranges up to kMarkSpan), else straight to minv
+    const bool slots = tb.by_marks;
+    if (slots) {
+      for (int i = lane; i <= vend - base; i += kWave) slot[i] = kNoKey;
+      wave_lds_sync();
+#pragma unroll
+      for (int u = 0; u < SUB; u++)
+        if (cross[u]) atomicMin(&slot[src[u] - base], make_key(wv[u], j[u]));
+      wave_lds_sync();
+    } else {
+#pragma unroll
+      for (int u = 0; u < SUB; u++)
+        if (cross[u]) min_key(minv, a[u], make_key(wv[u], j[u]), true);
+    }
+    // M's minimum: over the crossing positions whose neighbour is in M
+    {
+      u64 bigk = kNoKey;
+#pragma unroll
+      for (int u = 0; u < SUB; u++) {
+        const u64 key = make_key(wv[u], j[u]);
+        if (cross[u] && tom[u] && key < bigk) bigk = key;
+      }
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        const u64 o = __shfl_xor(bigk, off);
+        bigk = o < bigk ? o : bigk;
+      }
+      if (lane == 0 && bigk != kNoKey) atomicMin(&s_bigmin, bigk);
+    }
+    // records, with the first join's candidate flags.  The position that
+    // holds its source's minimum over the task (one per source: keys are
+    // distinct) writes it: a plain store for a component of its own whose
+    // list lies inside the task (the only contributor), else a guarded
+    // atomicMin to the source's root.
+    const int first_src = __builtin_amdgcn_readfirstlane(src[0]);
+    const int last_src = __builtin_amdgcn_readlane(src[SUB - 1], kWave - 1);
+    const bool whole = pbeg + TE <= edges;
+#pragma unroll
+    for (int u = 0; u < SUB; u++) {
+      const bool keep = cross[u] && (!lower[u] || tom[u]);
+      const u64 key = make_key(wv[u], j[u]);
+      const bool cand = !slots || key == slot[src[u] - base];
+      if (slots && cross[u] && cand) {
+        if (alone[u] && whole && src[u] != first_src && src[u] != last_src) minv[src[u]] = key;
+        else min_key(minv, a[u], key, true);
+      }
+      const int xa = cand ? (a[u] | kCandBit) : a[u];
       // records list the lower endpoint's root first, as k_init_wl<true>
-      const int4 rec = lower ? make_int4(b[u], a[u], wv, j) : make_int4(a[u], b[u], wv, j);
-      stage_push(st, keep[u], rec, seg, &s_fill);
+      const int4 rec = lower[u] ? make_int4(b[u] | kCandBit, xa, wv[u], j[u]) : make_int4(xa, b[u] | kCandBit, wv[u], j[u]);
+      stage_push(st, keep, rec, seg, &s_fill);
     }
+    if (slots) wave_lds_sync();  // the next task resets the slots
     g = gn;
     base = base_n;
     vend = vend_n;
 #pragma unroll
-    for (int u = 0; u < SUB; u++) {
-      nl[u] = nl_n[u];
-      w[u] = w_n[u];
-    }
+    for (int u = 0; u < SUB; u++) nl[u] = nl_n[u];
   }
   init_finish(st, seg, s_fill, segcnt, cnt, stats, ST_HEAVY);
+  if (threadIdx.x == 0 && s_bigmin != kNoKey) min_key(minv, bigroot, s_bigmin, true);
 }
 
 // Skip flags of the sparse heavy initialisation: task g (positions
@@ -617,10 +705,11 @@ __device__ __forceinline__ unsigned spread16(unsigned x)
 // parent[v] = root(v), and the vertex class map: two bits per vertex,
 // sixteen vertices per word, bit 0 = (root(v) == *big), bit 1 = (v is a
 // root).  Concurrent writers only shorten chains toward the same root, so
-// readers stay correct.
+// readers stay correct.  With hasmem (cleared by k_setup), every root
+// outside the marked component that has other members gets hasmem[root] = 1.
 __global__ __launch_bounds__(256) void k_flatten(int nodes, int* __restrict__ parent, int* __restrict__ cnt,
                                                  const int* __restrict__ big, unsigned* __restrict__ vclass,
-                                                 const int* __restrict__ dthr)
+                                                 const int* __restrict__ dthr, unsigned char* __restrict__ hasmem)
 {
   const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (v == 0) {
@@ -633,6 +722,7 @@ __global__ __launch_bounds__(256) void k_flatten(int nodes, int* __restrict__ pa
   if (v < nodes) {
     r = find_root((int)v, parent);
     parent[v] = r;
+    if (hasmem && r != (int)v && r != bigroot) hasmem[r] = 1;
   }
   const u64 mb = __ballot(r == bigroot && r >= 0);
   const u64 mr = __ballot(r >= 0 && r == (int)v);
@@ -784,7 +874,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_select(
     const int4* __restrict__ wl_in, const int* __restrict__ cnt_in, int4* __restrict__ wl_out,
     int* __restrict__ cnt_out, const int* __restrict__ parent, u64* __restrict__ minv, int guard, int combine,
     int dedup, long long* __restrict__ stats, int rounds_slot, u64* __restrict__ cand, const u64* __restrict__ dead,
-    ListView lv)
+    ListView lv, const int* __restrict__ fused)
 {
   constexpr int BT = WAVES * kWave;
   constexpr int TILE = BT * kSelItems;
@@ -795,6 +885,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_select(
   __shared__ int s_base;
   __shared__ u64 s_ptag[FIRST ? 1 : kPair];
   __shared__ u64 s_pkey[FIRST ? 1 : kPair];
+  if (FIRST && fused && *fused) return;  // the initialisation made this round's offers
   const int n_in = *cnt_in;
   if (n_in == 0) return;
   const long long span = list_span(lv, n_in);
@@ -834,8 +925,9 @@ __global__ __launch_bounds__(WAVES * kWave) void k_select(
       keep[k] = false;
       bool cx = false, cy = false;
       if (live[k]) {
-        const int a = FIRST ? r[k].x : find_root(r[k].x, parent);
-        const int c = FIRST ? r[k].y : find_root(r[k].y, parent);
+        // (bit 31 of an endpoint is a first-join candidate flag)
+        const int a = FIRST ? r[k].x : find_root(r[k].x & INT_MAX, parent);
+        const int c = FIRST ? r[k].y : find_root(r[k].y & INT_MAX, parent);
         if (a != c) {
           keep[k] = true;
           const u64 key = make_key(r[k].z, r[k].w);
@@ -986,17 +1078,23 @@ __device__ void tail_rounds(int4* cur, int n, int4* nxt, int* parent, u64* minv,
 // kernel3 away), hooks the two roots and marks inMST[j].  A reader racing
 // with a reset sees either the winner's key or the sentinel, neither equal
 // to its own key, so the outcome is the same as with a separate kernel3.
+// The phase's first join checks only candidate endpoints: those of the
+// candidate bitmasks written by the offer-only select, or, when *fused is
+// set (the initialisation made the offers), those flagged in bit 31 of the
+// record's endpoints; it then also counts the round that the skipped select
+// would have counted.
 // ---------------------------------------------------------------------------
 template <int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void k_join(
     int4* __restrict__ wl, int* __restrict__ cnt, int4* __restrict__ wl_other, int* __restrict__ cnt_other,
     int* __restrict__ parent, u64* __restrict__ minv, uint8_t* __restrict__ inMST, int tail_max,
     int* __restrict__ hflag, long long* __restrict__ stats, int rounds_slot, const u64* cand, u64* joined_mask,
-    ListView lv)
+    ListView lv, const int* __restrict__ fused)
 {
   constexpr int BT = WAVES * kWave;
   const int n = *cnt;
-  if (!lv.segcnt && n > 0 && n <= tail_max) {
+  const bool flags = fused && *fused;
+  if (!lv.segcnt && !flags && n > 0 && n <= tail_max) {
     if (blockIdx.x != 0) return;
     if (threadIdx.x == 0) {
       stats[ST_OUT_SUM] += n;
@@ -1012,6 +1110,10 @@ __global__ __launch_bounds__(WAVES * kWave) void k_join(
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *cnt_other = 0;
     stats[ST_OUT_SUM] += n;
+    if (flags && n > 0) {
+      stats[rounds_slot] += 1;
+      stats[ST_IN_SUM] += n;
+    }
     __hip_atomic_store(hflag, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
   if (n == 0) return;  // (segment counts are only valid for a non-empty list)
@@ -1023,14 +1125,18 @@ __global__ __launch_bounds__(WAVES * kWave) void k_join(
     // after the offer-only first round, only a candidate endpoint can have
     // this record as its minimum; later rounds check both endpoints
     bool cx = live, cy = live;
-    if (cand && live) {
+    if (cand && !flags && live) {
       cx = (cand[i / kWave] >> (i % kWave)) & 1ull;
       cy = (cand[cand_words + i / kWave] >> (i % kWave)) & 1ull;
     }
     bool joined = false;
     if (cx || cy) {
       const int4 r = wl[i];
-      const int x = r.x, y = r.y;
+      if (flags) {
+        cx = r.x < 0;
+        cy = r.y < 0;
+      }
+      const int x = r.x & INT_MAX, y = r.y & INT_MAX;
       const u64 key = make_key(r.z, r.w);
       const bool wx = cx && minv[x] == key;
       const bool wy = cy && minv[y] == key;
@@ -1072,12 +1178,12 @@ typedef void (*InitFn)(const int*, const int*, const int*, const int*, int, int,
                        const unsigned*, int4*, long long, int*, int*, int, long long*, const int*, const int*,
                        const int*, const int*, int);
 typedef void (*SparseFn)(const int*, const int*, const int*, const int*, int, int, const int*, const unsigned*,
-                         const unsigned char*, const int*, int, int4*, long long, int*, int*, int, long long*,
-                         const int*, const int*);
+                         const unsigned char*, const int*, int, const unsigned char*, u64*, int4*, long long, int*,
+                         int*, int, long long*, const int*, const int*);
 typedef void (*SelFn)(const int4*, const int*, int4*, int*, const int*, u64*, int, int, int, long long*, int, u64*,
-                      const u64*, ListView);
+                      const u64*, ListView, const int*);
 typedef void (*JoinFn)(int4*, int*, int4*, int*, int*, u64*, uint8_t*, int, int*, long long*, int, const u64*, u64*,
-                       ListView);
+                       ListView, const int*);
 
 InitFn init_fn(bool heavy, int sub)
 {
@@ -1227,6 +1333,7 @@ int ensure_buffers(emst_ctx* ctx, int32_t nodes, int32_t edges, long long wl_cap
   if ((rc = grow((void**)&ctx->d_cand, &ctx->cap_cand, 2 * z.cand_words, 8, "candidate bitmasks"))) return rc;
   if ((rc = grow((void**)&ctx->d_segcnt, &ctx->cap_seg, z.segs, 4, "segment counts"))) return rc;
   if ((rc = grow((void**)&ctx->d_skip, &ctx->cap_skip, z.gsrc, 1, "task skip flags"))) return rc;
+  if ((rc = grow((void**)&ctx->d_hasmem, &ctx->cap_hasmem, z.nodes, 1, "root member flags"))) return rc;
   return EMST_OK;
 }
 
@@ -1306,9 +1413,13 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
     wb.stats = ctx->d_stats;
     wb.segcnt = ctx->d_segcnt;
     wb.skip = ctx->d_skip;
+    wb.hasmem = ctx->d_hasmem;
   }
-  // the sparse heavy initialisation needs reverse positions and skip flags
-  const bool sparse = rix && rix->rev && wb.skip && !(cfg.flags & EMST_F_NO_SPARSE_HEAVY);
+  // the sparse heavy initialisation needs reverse positions, skip flags and
+  // the member flags of the roots
+  const bool sparse = rix && rix->rev && wb.skip && wb.hasmem && !(cfg.flags & EMST_F_NO_SPARSE_HEAVY) &&
+                      sparse_lds_bytes(std::max(64, cfg.stage_recs)) <= ctx->lds_per_block;
+  unsigned char* const hasmem = sparse ? wb.hasmem : nullptr;
   const WorkSizes z = work_sizes(ctx->cus, cfg, nodes, edges, wl_cap);
   const bool learned = use_filter == EMST_FILTER_LEARNED && !vt;
   const int T = use_filter == EMST_FILTER_PLAIN ? threshold : INT_MAX;
@@ -1354,7 +1465,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
   {
     const int blocks = (int)(((long long)nodes + 255) / 256);
     hipLaunchKernelGGL(k_setup, dim3(blocks), dim3(256), 0, s, nodes, d_nindex, wb.parent, wb.minv, wb.gsrc, d_cnt,
-                       d_stats);
+                       d_stats, hasmem);
     const long long z16 = ((long long)edges + 15) / 16;
     const int zb = (int)std::min<long long>(std::max<long long>((z16 + 255) / 256, 1), (long long)ctx->cus * 16);
     hipLaunchKernelGGL(k_zero, dim3(zb), dim3(256), 0, s, d_inMST, (long long)edges);
@@ -1383,7 +1494,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       const int blocks = (int)(((long long)nodes + 255) / 256);
       hipLaunchKernelGGL(k_pick_big, dim3(1), dim3(kWave), 0, s, nodes, edges, d_nlist, wb.parent, &d_cnt[2], dthr);
       hipLaunchKernelGGL(k_flatten, dim3(blocks), dim3(256), 0, s, nodes, wb.parent, d_cnt, &d_cnt[2], wb.vclass,
-                         dthr);
+                         dthr, hasmem);
       if (sparse && edges > 0) {
         const long long ngroups = ((long long)edges + kStartEdges - 1) / kStartEdges;
         hipLaunchKernelGGL(k_task_skip, dim3((unsigned)((ngroups + 255) / 256)), dim3(256), 0, s, ngroups, nodes,
@@ -1394,13 +1505,14 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       if (!fixed) HIPCHK(hipEventRecord(ctx->ev[2 + 2 * phase], s));
       // With reverse positions both heavy passes are enqueued; the marked
       // component's share of the adjacency (k_pick_big) lets one of them run
-      // and the other return at once.
+      // and the other return at once.  The sparse pass also makes the first
+      // round's offers, and then the offer-only select returns at once too.
       if (phase == 1 && sparse) {
         const SparseFn f = sparse_fn();
-        const size_t sparse_lds = init_lds_bytes(stage_cap, kSparseSub * kWave);
-        hipLaunchKernelGGL(f, dim3(z.init_grid), dim3(kInitWaves * kWave), sparse_lds, s, d_nindex, d_nlist, d_eweight,
-                           wb.gsrc, nodes, edges, wb.parent, wb.vclass, wb.skip, rix->rev, rix->mirrored ? 1 : 0,
-                           wl[0], z.seg_cap, wb.segcnt, &d_cnt[0], stage_cap, d_stats, dthr, &d_cnt[2]);
+        hipLaunchKernelGGL(f, dim3(z.init_grid), dim3(kInitWaves * kWave), sparse_lds_bytes(stage_cap), s, d_nindex,
+                           d_nlist, d_eweight, wb.gsrc, nodes, edges, wb.parent, wb.vclass, wb.skip, rix->rev,
+                           rix->mirrored ? 1 : 0, hasmem, wb.minv, wl[0], z.seg_cap, wb.segcnt, &d_cnt[0], stage_cap,
+                           d_stats, dthr, &d_cnt[2]);
       }
       const InitFn f = init_fn(phase == 1, sub);
       hipLaunchKernelGGL(f, dim3(z.init_grid), dim3(kInitWaves * kWave), init_lds, s, d_nindex, d_nlist, d_eweight,
@@ -1412,6 +1524,8 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       init_launches++;
     }
     const int rounds_slot = phase == 0 ? ST_ROUNDS0 : ST_ROUNDS1;
+    // set on the device when the sparse heavy initialisation ran
+    const int* const fused = (phase == 1 && sparse) ? &d_cnt[3] : nullptr;
     // Device-driven rounds with a bounded look-ahead (none in host-sync
     // mode).  The phase's first select and join and the select after them
     // read the initialisation's segments; later lists are dense.
@@ -1430,15 +1544,15 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       if (r == 0)
         hipLaunchKernelGGL(sel_first, dim3(grid), dim3(waves * kWave), 0, s, wl[0], &d_cnt[0], wl[1], &d_cnt[1],
                            wb.parent, wb.minv, guard, combine, dedup, d_stats, rounds_slot, wb.cand,
-                           (const u64*)nullptr, lv);
+                           (const u64*)nullptr, lv, fused);
       // the first join reads the candidate bitmasks; every join marks what it joined
       hipLaunchKernelGGL(join, dim3(grid), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt], &d_cnt[nxt],
                          wb.parent, wb.minv, d_inMST, tail_max, ctx->d_flag + slot, d_stats, rounds_slot,
-                         (const u64*)(r == 0 ? wb.cand : nullptr), wb.cand, lv);
+                         (const u64*)(r == 0 ? wb.cand : nullptr), wb.cand, lv, r == 0 ? fused : nullptr);
       // each select drops the records the join before it joined
       hipLaunchKernelGGL(sel, dim3(grid), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt], &d_cnt[nxt],
                          wb.parent, wb.minv, guard, combine, dedup, d_stats, rounds_slot, (u64*)nullptr,
-                         (const u64*)wb.cand, lv);
+                         (const u64*)wb.cand, lv, (const int*)nullptr);
       HIPCHK(hipGetLastError());
       slot++;
       if (fixed) {
@@ -1571,7 +1685,7 @@ void emst_destroy(emst_ctx* ctx)
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   void* ptrs[] = {ctx->d_cand,   ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, ctx->d_inMST, ctx->d_parent, ctx->d_segcnt,
-                  ctx->d_skip,   ctx->d_rev,
+                  ctx->d_skip,   ctx->d_hasmem, ctx->d_rev,
                   ctx->d_minv,   ctx->d_wl[0], ctx->d_wl[1],   ctx->d_gsrc,  ctx->d_cnt,
                   ctx->d_stats,  ctx->d_vclass, ctx->d_mlp16,  ctx->d_thr,   ctx->d_q,
                   ctx->d_feat,   ctx->d_samples, ctx->d_lad[0], ctx->d_lad[1], ctx->d_slist,
