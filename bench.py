@@ -179,18 +179,26 @@ def run_single(a, emst):
     # weights 4*m_up, records 16 B; the heavy pass adds one root read per
     # vertex and per heavy upper-direction edge.  The sparse heavy pass does
     # the same job touching less (it skips the marked component's
-    # adjacency); it is priced at the same bytes.
+    # adjacency); it is priced at the same bytes.  An initialisation that also
+    # makes its phase's first round of offers is priced with that round too:
+    # the reference's first kernel1 reads every record and offers its key to
+    # both roots, 16 + 2 * 8 bytes per record.
     heavy_up = m_up - stl.light
     heavy_bytes = 4 * (n + 1) + 4 * m + 4 * m_up + 4 * n + 4 * heavy_up + 16 * stl.heavy
     light_bytes = 4 * (n + 1) + 4 * m + 4 * m_up + 16 * stl.light
+    if stl.sparse_heavy:
+        heavy_bytes += 32 * stl.heavy
+    if stl.light_offers:
+        light_bytes += 32 * stl.light
+    light_name = "k_init_light_offers<4>" if stl.light_offers else "k_init_wl<false, 8>"
     heavy_ms = statistics.mean(x.init_heavy_ms for x in st_l)
     light_ms = statistics.mean(x.init_light_ms for x in st_l)
     tr = read_traffic(f"traffic_{a.workload}.json") if a.scale is None else None
     heavy_name = "k_init_heavy_sparse<4>" if stl.sparse_heavy else "k_init_wl<true, 8>"
-    kernels = {"light": {"kernel": "k_init_wl<false, 8> (light worklist initialisation)", "avg_launch_ms": light_ms,
+    kernels = {"light": {"kernel": f"{light_name} (light worklist initialisation)", "avg_launch_ms": light_ms,
                          "alg_bytes_per_launch": light_bytes,
                          "achieved": light_bytes / (light_ms * 1e-3) / 1e9,
-                         "traffic": tr.get("k_init_wl<false, 8>") if tr else None}}
+                         "traffic": tr.get(light_name) if tr else None, "first_offers": bool(stl.light_offers)}}
     if stl.filter_used:
         kernels["heavy"] = {"kernel": f"{heavy_name} (heavy worklist initialisation)", "avg_launch_ms": heavy_ms,
                             "alg_bytes_per_launch": heavy_bytes,

@@ -87,6 +87,7 @@ typedef struct {
   int32_t filter_used;       /* 1 when the two-phase light/heavy path ran              */
   float learned_q;           /* quantile predicted by the network, -1 if not used      */
   int32_t sparse_heavy;      /* 1 when the sparse heavy initialisation ran             */
+  int32_t light_offers;      /* 1 when the light initialisation made the first offers  */
 } emst_stats;
 
 /* Tunables.  Zero in a field selects the built-in default. */
@@ -120,6 +121,13 @@ typedef struct {
  * largest light-phase component and skips the rest of the CSR.  Results,
  * records and round counts are the same either way (ablation). */
 #define EMST_F_NO_SPARSE_HEAVY 2048
+/* Make the light phase's first offers in a separate select.  By default,
+ * when the loaded graph's lists are sorted and symmetric with the same
+ * weight on both copies of every edge, the light initialisation also makes
+ * the first round's offers (every vertex is then a component of its own and
+ * sees all its edges in its own list), and that round's select is skipped.
+ * Results, records and round counts are the same either way (ablation). */
+#define EMST_F_NO_LIGHT_OFFERS 4096
 
 /* The de-optimization ladder (reference: Deoptimizations/ECL_MST_No*.cu,
  * removing one optimization per file).  EMST_F_LADDER runs the reference's

@@ -85,6 +85,9 @@ struct emst_ctx {
 
 namespace emst {
 
+// Device counter words of one solve (WorkBufs::cnt).
+constexpr int kCntWords = 8;
+
 // Device work buffers of one solve: the context's own (grown on demand) or
 // carved from the library's static scratch array.
 struct WorkBufs {
@@ -94,7 +97,7 @@ struct WorkBufs {
   int* gsrc = nullptr;          // start vertex of every initialisation task
   int4* wl[2] = {nullptr, nullptr};
   u64* cand = nullptr;          // two bitmasks of cand_words words
-  int* cnt = nullptr;           // [list 0 size, list 1 size, marked component, -]
+  int* cnt = nullptr;           // kCntWords: [list 0 size, list 1 size, marked component, sparse, 1, -, -, -]
   long long* stats = nullptr;   // ST_COUNT counters
   int* segcnt = nullptr;        // records per initialisation segment
   unsigned char* skip = nullptr;  // per-task skip flags (sparse heavy initialisation)
@@ -177,6 +180,6 @@ int noalloc_setup(int device);
 constexpr int kLadderFlags = EMST_F_LADDER | EMST_F_NO_WARP | EMST_F_NO_IMPLICIT_PC | EMST_F_NO_ONE_DIRECTION |
                              EMST_F_NO_TUPLES | EMST_F_NO_DATA_DRIVEN | EMST_F_NO_EDGE_BASED;
 constexpr int kAllFlags = EMST_F_HOST_SYNC | EMST_F_NO_GUARD | EMST_F_NO_COMBINE | EMST_F_NO_PAIR_DEDUP |
-                          EMST_F_NO_SPARSE_HEAVY | kLadderFlags;
+                          EMST_F_NO_SPARSE_HEAVY | EMST_F_NO_LIGHT_OFFERS | kLadderFlags;
 
 }  // namespace emst

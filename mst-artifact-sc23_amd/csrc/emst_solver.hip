@@ -54,7 +54,7 @@ constexpr int kHflagSlots = 512;  // host-visible round results per solve
 constexpr int kSelItems = 4;      // records per thread per k_select tile
 constexpr int kTbl = 1024;        // LDS min-combining table entries
 constexpr int kPair = 1024;       // LDS parallel-record table entries (k_select after the first round)
-constexpr int kReportWords = ST_COUNT + 4;  // counters, threshold, filter flag, quantile bits, sparse heavy
+constexpr int kReportWords = ST_COUNT + 5;  // counters, threshold, filter flag, quantile bits, sparse heavy, light offers
 
 // Dynamic LDS of an initialisation workgroup: per-wave staging and marks,
 // then the workgroup's fill count (16 bytes keep the size a multiple of 16).
@@ -80,6 +80,7 @@ __global__ __launch_bounds__(256) void k_setup(int nodes, const int* __restrict_
     cnt[1] = 0;
     cnt[2] = 0;
     cnt[3] = 0;  // the sparse heavy decision, set by k_pick_big
+    cnt[4] = 0;  // the light initialisation's choice, set by k_pick_light
     for (int i = 0; i < ST_COUNT; i++) stats[i] = 0;
   }
   if (v < nodes) {
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
     const unsigned* __restrict__ vclass, int4* __restrict__ wl, long long seg_cap, int* __restrict__ segcnt,
     int* __restrict__ cnt, int stage_cap, long long* __restrict__ stats, const int* __restrict__ dthr,
     const int* __restrict__ vgid, const int* __restrict__ gthr, const int* __restrict__ bigroot_p,
-    int sparse_ready)
+    const int* __restrict__ alt_ran)
 {
   constexpr int TE = SUB * kWave;
   // dynamic LDS: per-wave staging, per-wave marks, the workgroup's fill count
@@ -263,7 +264,7 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
     T = dthr[0];
   }
   // the sparse pass was enqueued too and the marked component chose it
-  if (HEAVY && sparse_ready && bigroot_p[1]) return;
+  if (alt_ran && *alt_ran) return;
   if (threadIdx.x == 0) s_fill = 0;
   __syncthreads();
   int4* const seg = wl + (long long)blockIdx.x * seg_cap;
@@ -574,6 +575,153 @@ __global__ __launch_bounds__(kInitWaves * kWave) __attribute__((amdgpu_waves_per
   }
   init_finish(st, seg, s_fill, segcnt, cnt, stats, ST_HEAVY);
   if (threadIdx.x == 0 && s_bigmin != kNoKey) min_key(minv, bigroot, s_bigmin, true);
+}
+
+// ---------------------------------------------------------------------------
+// Light initialisation fused with the light phase's first round of offers,
+// for sorted, symmetric lists with reverse positions and mirrored weights.
+// The records are k_init_wl<false>'s: {v, u, w, p} for every upper-direction
+// position p of v (u = nlist[p] > v) with w <= T (ECL_MST.cu:170-176).  When
+// the light phase starts every vertex is a component of its own, and v's
+// list holds every edge incident to v, so v's first offer (kernel1's
+// atomicMin, ECL_MST.cu:222-223) is the minimum key over v's light
+// positions, each keyed by its upper-direction copy: (w, p) for an upper
+// position, (w, rev[p]) for a lower one (mirrored weights: both copies carry
+// w, so the light test needs no reverse position).  The minimum is reduced
+// per task in LDS slots, one per source vertex, and the position holding it
+// writes minv[v]: a plain store when v's whole list lies inside the task,
+// else a guarded atomicMin.
+// Records carry the first join's candidate flags in bit 31 of their
+// endpoints, as in the sparse heavy pass: v's side only if the key is the
+// task's minimum for v, u's side always.  Tasks are kStartEdges positions,
+// wave w of workgroup b takes tasks b*kInitWaves + w + k*nwaves, and the
+// next task's stream loads (lists, weights and reverse positions: light
+// edges are dense where this pass runs) are issued before the current one is
+// processed.
+// Runs when *decide is set (k_pick_light); k_init_wl<false> runs otherwise.
+// ---------------------------------------------------------------------------
+template <int SUB>
+__global__ __launch_bounds__(kInitWaves * kWave) __attribute__((amdgpu_waves_per_eu(5))) void k_init_light_offers(
+    const int* __restrict__ nindex, const int* __restrict__ nlist, const int* __restrict__ eweight,
+    const int* __restrict__ gsrc, int nodes, int edges, int T, const int* __restrict__ rev, u64* __restrict__ minv,
+    int4* __restrict__ wl, long long seg_cap, int* __restrict__ segcnt, int* __restrict__ cnt, int stage_cap,
+    long long* __restrict__ stats, const int* __restrict__ dthr, const int* __restrict__ decide)
+{
+  constexpr int TE = SUB * kWave;
+  static_assert(TE == kStartEdges, "light offer tasks are start-table blocks");
+  constexpr int NSLOT = kMarkSpan + 1;
+  extern __shared__ int4 lds_stage[];
+  int* const tail = reinterpret_cast<int*>(lds_stage + kInitWaves * stage_cap) + kInitWaves * TE;
+  int& s_fill = tail[0];
+  if (!*decide) return;
+  if (dthr) T = dthr[0];
+  if (threadIdx.x == 0) s_fill = 0;
+  __syncthreads();
+  int4* const seg = wl + (long long)blockIdx.x * seg_cap;
+  const int lane = lane_id();
+  const int wid = threadIdx.x / kWave;
+  Stage st{lds_stage + wid * stage_cap, stage_cap, 0};
+  int* marks = reinterpret_cast<int*>(lds_stage + kInitWaves * stage_cap) + wid * TE;
+  u64* slot = reinterpret_cast<u64*>(tail + 4) + wid * NSLOT;
+  const long long ngroups = ((long long)edges + TE - 1) / TE;
+  const long long nwaves = (long long)gridDim.x * kInitWaves;
+  auto load_task = [&](long long t, int (&nl_t)[SUB], int (&w_t)[SUB], int (&r_t)[SUB], int& b, int& e) {
+    b = gsrc[t];
+    e = t + 1 < ngroups ? gsrc[t + 1] : nodes - 1;
+#pragma unroll
+    for (int u = 0; u < SUB; u++) {
+      const long long p = t * TE + u * kWave + lane;
+      nl_t[u] = p < edges ? ld_stream(&nlist[p]) : -1;
+      w_t[u] = p < edges ? ld_stream(&eweight[p]) : 0;
+      r_t[u] = p < edges ? ld_stream(&rev[p]) : 0;
+    }
+  };
+  long long g = (long long)blockIdx.x * kInitWaves + wid;
+  int nl[SUB], w[SUB], rv[SUB], base = 0, vend = 0;
+  if (g < ngroups) load_task(g, nl, w, rv, base, vend);
+  for (; g < ngroups; g += nwaves) {
+    const long long gbeg = g * TE;
+    TaskBounds tb;
+    task_bounds_load(tb, nindex, nodes, base, vend, lane);
+    const long long gn = g + nwaves;
+    int nl_n[SUB], w_n[SUB], rv_n[SUB], base_n = 0, vend_n = 0;
+    if (gn < ngroups) load_task(gn, nl_n, w_n, rv_n, base_n, vend_n);
+    int src[SUB];
+    task_sources<SUB>(src, tb, gbeg, base, edges, nodes, nindex, marks, lane);
+    const int pbeg = (int)gbeg;  // positions fit in int (edges < 2^31)
+    // light positions and their keys
+    bool light[SUB], lower[SUB];
+    int j[SUB];
+#pragma unroll
+    for (int u = 0; u < SUB; u++) {
+      const int p = pbeg + u * kWave + lane;
+      lower[u] = nl[u] < src[u];
+      light[u] = p < edges && nl[u] != src[u] && w[u] <= T;
+      j[u] = (light[u] && lower[u]) ? rv[u] : p;
+    }
+    // each source's minimum over this task: LDS slots by vertex (vertex
+    // ranges up to kMarkSpan), else straight to minv
+    const bool slots = tb.by_marks;
+    if (slots) {
+      for (int i = lane; i <= vend - base; i += kWave) slot[i] = kNoKey;
+      wave_lds_sync();
+#pragma unroll
+      for (int u = 0; u < SUB; u++)
+        if (light[u]) atomicMin(&slot[src[u] - base], make_key(w[u], j[u]));
+      wave_lds_sync();
+    } else {
+#pragma unroll
+      for (int u = 0; u < SUB; u++)
+        if (light[u]) min_key(minv, src[u], make_key(w[u], j[u]), true);
+    }
+    // records with candidate flags; the position holding its source's
+    // minimum (one per source: keys are distinct) writes it
+    const int first_src = __builtin_amdgcn_readfirstlane(src[0]);
+    const int last_src = __builtin_amdgcn_readlane(src[SUB - 1], kWave - 1);
+    const bool whole = pbeg + TE <= edges;
+#pragma unroll
+    for (int u = 0; u < SUB; u++) {
+      const u64 key = make_key(w[u], j[u]);
+      const bool cand = !slots || key == slot[src[u] - base];
+      if (slots && light[u] && cand) {
+        if (whole && src[u] != first_src && src[u] != last_src) minv[src[u]] = key;
+        else min_key(minv, src[u], key, true);
+      }
+      const int x = cand ? (src[u] | kCandBit) : src[u];
+      stage_push(st, light[u] && !lower[u], make_int4(x, nl[u] | kCandBit, w[u], j[u]), seg, &s_fill);
+    }
+    if (slots) wave_lds_sync();  // the next task resets the slots
+    base = base_n;
+    vend = vend_n;
+#pragma unroll
+    for (int u = 0; u < SUB; u++) {
+      nl[u] = nl_n[u];
+      w[u] = w_n[u];
+      rv[u] = rv_n[u];
+    }
+  }
+  init_finish(st, seg, s_fill, segcnt, cnt, stats, ST_LIGHT);
+}
+
+// Chooses between the two light initialisations of a graph with reverse
+// positions: the pass that also makes the first offers spends more per
+// position and saves the first select, which costs per light record, so it
+// pays when light edges are a large share of the adjacency.  *out = 1 when
+// at least a tenth of 1024 evenly spaced positions are light (w <= T).
+__global__ __launch_bounds__(64) void k_pick_light(int edges, const int* __restrict__ eweight, int T,
+                                                   const int* __restrict__ dthr, int* __restrict__ out)
+{
+  if (dthr) T = dthr[0];
+  const int lane = lane_id();
+  int c = 0;
+  for (int k = 0; k < 16; k++) {
+    const long long i = k * kWave + lane;
+    const long long p = i * edges / 1024 + edges / 2048;
+    c += p < edges && eweight[p] <= T;
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off);
+  if (lane == 0) *out = edges > 0 && 10 * c >= 1024;
 }
 
 // Skip flags of the sparse heavy initialisation: task g (positions
@@ -1162,7 +1310,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_join(
 // event without further copies.
 __global__ __launch_bounds__(64) void k_report(const long long* __restrict__ stats, const int* __restrict__ thr,
                                                const float* __restrict__ q, const int* __restrict__ sparse,
-                                               long long* __restrict__ out)
+                                               const int* __restrict__ light, long long* __restrict__ out)
 {
   const int t = threadIdx.x;
   long long v = 0;
@@ -1170,16 +1318,19 @@ __global__ __launch_bounds__(64) void k_report(const long long* __restrict__ sta
   else if (t < ST_COUNT + 2) v = thr ? thr[t - ST_COUNT] : 0;
   else if (t == ST_COUNT + 2) v = q ? (long long)__float_as_uint(q[0]) : 0;
   else if (t == ST_COUNT + 3) v = sparse ? *sparse : 0;
+  else if (t == ST_COUNT + 4) v = light ? *light : 0;
   if (t < kReportWords) __hip_atomic_store(&out[t], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // ---- runtime selection of template instances -------------------------------
 typedef void (*InitFn)(const int*, const int*, const int*, const int*, int, int, int, const int*,
                        const unsigned*, int4*, long long, int*, int*, int, long long*, const int*, const int*,
-                       const int*, const int*, int);
+                       const int*, const int*, const int*);
 typedef void (*SparseFn)(const int*, const int*, const int*, const int*, int, int, const int*, const unsigned*,
                          const unsigned char*, const int*, int, const unsigned char*, u64*, int4*, long long, int*,
                          int*, int, long long*, const int*, const int*);
+typedef void (*LightFn)(const int*, const int*, const int*, const int*, int, int, int, const int*, u64*, int4*,
+                        long long, int*, int*, int, long long*, const int*, const int*);
 typedef void (*SelFn)(const int4*, const int*, int4*, int*, const int*, u64*, int, int, int, long long*, int, u64*,
                       const u64*, ListView, const int*);
 typedef void (*JoinFn)(int4*, int*, int4*, int*, int*, u64*, uint8_t*, int, int*, long long*, int, const u64*, u64*,
@@ -1200,6 +1351,11 @@ InitFn init_fn(bool heavy, int sub)
 SparseFn sparse_fn()
 {
   return k_init_heavy_sparse<kSparseSub>;
+}
+
+LightFn light_offers_fn()
+{
+  return k_init_light_offers<kSparseSub>;
 }
 
 SelFn sel_fn(bool first, int waves)
@@ -1420,6 +1576,11 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
   const bool sparse = rix && rix->rev && wb.skip && wb.hasmem && !(cfg.flags & EMST_F_NO_SPARSE_HEAVY) &&
                       sparse_lds_bytes(std::max(64, cfg.stage_recs)) <= ctx->lds_per_block;
   unsigned char* const hasmem = sparse ? wb.hasmem : nullptr;
+  // the light initialisation that also makes the first offers needs every
+  // vertex's list to hold all its edges with their upper copies' weights;
+  // with it both light passes are enqueued and k_pick_light picks one
+  const bool light_offers = rix && rix->rev && rix->mirrored && !vt && !(cfg.flags & EMST_F_NO_LIGHT_OFFERS) &&
+                            sparse_lds_bytes(std::max(64, cfg.stage_recs)) <= ctx->lds_per_block;
   const WorkSizes z = work_sizes(ctx->cus, cfg, nodes, edges, wl_cap);
   const bool learned = use_filter == EMST_FILTER_LEARNED && !vt;
   const int T = use_filter == EMST_FILTER_PLAIN ? threshold : INT_MAX;
@@ -1507,6 +1668,12 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       // component's share of the adjacency (k_pick_big) lets one of them run
       // and the other return at once.  The sparse pass also makes the first
       // round's offers, and then the offer-only select returns at once too.
+      if (phase == 0 && light_offers) {
+        hipLaunchKernelGGL(k_pick_light, dim3(1), dim3(kWave), 0, s, edges, d_eweight, T, dthr, &d_cnt[4]);
+        hipLaunchKernelGGL(light_offers_fn(), dim3(z.init_grid), dim3(kInitWaves * kWave), sparse_lds_bytes(stage_cap),
+                           s, d_nindex, d_nlist, d_eweight, wb.gsrc, nodes, edges, T, rix->rev, wb.minv, wl[0],
+                           z.seg_cap, wb.segcnt, &d_cnt[0], stage_cap, d_stats, dthr, &d_cnt[4]);
+      }
       if (phase == 1 && sparse) {
         const SparseFn f = sparse_fn();
         hipLaunchKernelGGL(f, dim3(z.init_grid), dim3(kInitWaves * kWave), sparse_lds_bytes(stage_cap), s, d_nindex,
@@ -1518,14 +1685,15 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       hipLaunchKernelGGL(f, dim3(z.init_grid), dim3(kInitWaves * kWave), init_lds, s, d_nindex, d_nlist, d_eweight,
                          wb.gsrc, nodes, edges, T, wb.parent, wb.vclass, wl[0], z.seg_cap, wb.segcnt, &d_cnt[0],
                          stage_cap, d_stats, dthr, phase == 0 ? vgid : nullptr, phase == 0 ? gthr : nullptr,
-                         &d_cnt[2], (phase == 1 && sparse) ? 1 : 0);
+                         &d_cnt[2], phase == 1 ? (sparse ? &d_cnt[3] : nullptr) : (light_offers ? &d_cnt[4] : nullptr));
       if (!fixed) HIPCHK(hipEventRecord(ctx->ev[3 + 2 * phase], s));
       HIPCHK(hipGetLastError());
       init_launches++;
     }
     const int rounds_slot = phase == 0 ? ST_ROUNDS0 : ST_ROUNDS1;
-    // set on the device when the sparse heavy initialisation ran
-    const int* const fused = (phase == 1 && sparse) ? &d_cnt[3] : nullptr;
+    // set on the device when the phase's initialisation made its first
+    // offers (the light offers pass or the sparse heavy pass ran)
+    const int* const fused = phase == 0 ? (light_offers ? &d_cnt[4] : nullptr) : (sparse ? &d_cnt[3] : nullptr);
     // Device-driven rounds with a bounded look-ahead (none in host-sync
     // mode).  The phase's first select and join and the select after them
     // read the initialisation's segments; later lists are dense.
@@ -1580,7 +1748,8 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
   if (fixed) return EMST_OK;  // captured: the caller times and reads back
   if (stats) {
     hipLaunchKernelGGL(k_report, dim3(1), dim3(64), 0, s, d_stats, learned ? ctx->d_thr : nullptr,
-                       learned ? ctx->d_q : nullptr, sparse ? &d_cnt[3] : nullptr, ctx->d_report);
+                       learned ? ctx->d_q : nullptr, sparse ? &d_cnt[3] : nullptr, light_offers ? &d_cnt[4] : nullptr,
+                       ctx->d_report);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(ctx->ev[1], s));
@@ -1621,6 +1790,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
     stats->round_in_sum = hs[ST_IN_SUM];
     stats->round_out_sum = hs[ST_OUT_SUM];
     stats->sparse_heavy = (int32_t)hs[ST_COUNT + 3];
+    stats->light_offers = (int32_t)hs[ST_COUNT + 4];
   }
   return EMST_OK;
 }
@@ -1648,7 +1818,7 @@ int emst_create(int device, emst_ctx** out)
   ctx->lds_per_block = prop.sharedMemPerBlock;
   bool ok = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) == hipSuccess;
   ok = ok && noalloc_setup(device) == EMST_OK;
-  ok = ok && hipMalloc((void**)&ctx->d_cnt, 16) == hipSuccess;
+  ok = ok && hipMalloc((void**)&ctx->d_cnt, kCntWords * sizeof(int)) == hipSuccess;
   ok = ok && hipMalloc((void**)&ctx->d_stats, ST_COUNT * sizeof(long long)) == hipSuccess;
   ok = ok && hipHostMalloc((void**)&ctx->h_flag, kHflagSlots * sizeof(int), hipHostMallocMapped) == hipSuccess;
   ok = ok && hipHostGetDevicePointer((void**)&ctx->d_flag, ctx->h_flag, 0) == hipSuccess;
@@ -1670,6 +1840,8 @@ int emst_create(int device, emst_ctx** out)
     }
     ok = ok && hipFuncSetAttribute((const void*)sparse_fn(), hipFuncAttributeMaxDynamicSharedMemorySize, dyn) ==
                    hipSuccess;
+    ok = ok && hipFuncSetAttribute((const void*)light_offers_fn(), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   dyn) == hipSuccess;
   }
   if (!ok) {
     emst_destroy(ctx);

@@ -72,7 +72,7 @@ extern "C" int emst_solve_device_noalloc(emst_ctx* ctx, int32_t nodes, int32_t e
   const WorkSizes z = work_sizes(ctx->cus, ctx->cfg, nodes, edges, edges);
   const size_t sz[] = {(size_t)z.nodes * 4,         (size_t)z.nodes * 8,          (size_t)z.vclass_words * 4,
                        (size_t)z.gsrc * 4,          (size_t)z.wl[0] * 16,         (size_t)z.wl[1] * 16,
-                       (size_t)z.cand_words * 2 * 8, (size_t)4 * 4,               (size_t)ST_COUNT * 8,
+                       (size_t)z.cand_words * 2 * 8, (size_t)kCntWords * 4,               (size_t)ST_COUNT * 8,
                        (size_t)z.segs * 4};
   size_t off[sizeof(sz) / sizeof(sz[0])];
   size_t total = 0;

@@ -345,6 +345,41 @@ def test_sparse_heavy_matches_full_pass(solver):
     assert ran >= 20  # the sparse pass ran wherever the marked component holds most positions
 
 
+def test_light_offers_match_separate_select(solver):
+    """With sorted, symmetric lists and the same weight on both copies of an
+    edge, the light initialisation also makes the light phase's first offers
+    (every vertex is a component of its own and sees all its edges in its own
+    list) and that round's select is skipped: bytes, L, H, rounds and, with
+    parallel records kept, every round's worklist sizes are the reference
+    schedule's, and the same as with EMST_F_NO_LIGHT_OFFERS; filtered,
+    unfiltered and learned.  The pass is chosen on the device when light
+    edges are at least a tenth of the sampled positions: always unfiltered,
+    and at a threshold at the third of the weights."""
+    ran = 0
+    cases = _sparse_cases()
+    for name, g in cases:
+        uf, T = emst.plain_threshold(g)
+        w = np.sort(g.eweight) if g.edges else np.zeros(1, np.int32)
+        for f, T2 in ((True, T), (True, int(w[len(w) // 3])), (False, emst.INT_MAX)):
+            got, st = check(solver, g, f, T2)
+            got2, st2 = with_flags(solver, emst.F_NO_LIGHT_OFFERS, lambda: solver.gpu_mst(g, f, T2))
+            assert np.array_equal(got, got2), name
+            assert (st.light, st.heavy, st.light_rounds, st.heavy_rounds) == \
+                (st2.light, st2.heavy, st2.light_rounds, st2.heavy_rounds), name
+            assert st2.light_offers == 0
+            if T2 != T:
+                assert st.light_offers == (1 if g.edges else 0), name
+            ran += st.light_offers
+        solver.load(g)
+        st = solver.solve(emst.FILTER_LEARNED, 0)
+        got = solver.fetch()
+        st2 = with_flags(solver, emst.F_NO_LIGHT_OFFERS, lambda: solver.solve(emst.FILTER_LEARNED, 0))
+        assert np.array_equal(got, solver.fetch()), name
+        assert (st.light, st.heavy, st.light_rounds, st.heavy_rounds) == \
+            (st2.light, st2.heavy, st2.light_rounds, st2.heavy_rounds), name
+    assert ran >= 2 * len(cases)
+
+
 def test_sparse_heavy_unmirrored_weights(solver):
     """Symmetric, sorted lists whose two copies of an edge carry different
     weights: the reference keys an edge by its upper-direction copy's weight,
@@ -358,9 +393,10 @@ def test_sparse_heavy_unmirrored_weights(solver):
         ew[low] = rng.integers(1, 1000, int(low.sum()))
         g = G(ni, nl, ew)
         for T in (50, 300):
-            got, _ = check(solver, g, True, T)
+            got, st = check(solver, g, True, T)
             got2, _ = with_flags(solver, emst.F_NO_SPARSE_HEAVY, lambda: solver.gpu_mst(g, True, T))
             assert np.array_equal(got, got2)
+            assert st.light_offers == 0  # the light offers need both copies' weights equal
 
 
 def test_sparse_heavy_falls_back_on_unsorted_or_one_way_lists(solver):
