@@ -67,7 +67,8 @@ size_t init_lds_bytes(int stage_recs, int task_edges)
 // k_setup: initPM (ECL_MST.cu:147-154) plus the task start table: for every
 // kStartEdges-edge block, the vertex v whose adjacency range holds the
 // block's first edge writes it (a task of k blocks reads every k-th entry).
-// Zero-degree vertices own no edge and write nothing.
+// Zero-degree vertices own no edge and write nothing.  A vertex that starts
+// more than 16 blocks (a hub) has its entries written by its whole wave.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_setup(int nodes, const int* __restrict__ nindex,
                                                int* __restrict__ parent, u64* __restrict__ minv,
@@ -83,16 +84,24 @@ __global__ __launch_bounds__(256) void k_setup(int nodes, const int* __restrict_
     cnt[4] = 0;  // the light initialisation's choice, set by k_pick_light
     for (int i = 0; i < ST_COUNT; i++) stats[i] = 0;
   }
+  int g0 = 0, g1 = 0;
   if (v < nodes) {
     parent[v] = (int)v;
     minv[v] = kNoKey;
     if (hasmem) hasmem[v] = 0;
     const int beg = nindex[v], end = nindex[v + 1];
     if (end > beg) {
-      const long long g0 = ((long long)beg + kStartEdges - 1) / kStartEdges;
-      const long long g1 = ((long long)end + kStartEdges - 1) / kStartEdges;
-      for (long long g = g0; g < g1; g++) gsrc[g] = (int)v;
+      g0 = (int)(((long long)beg + kStartEdges - 1) / kStartEdges);
+      g1 = (int)(((long long)end + kStartEdges - 1) / kStartEdges);
     }
+  }
+  const bool hub = g1 - g0 > 16;
+  if (!hub)
+    for (int g = g0; g < g1; g++) gsrc[g] = (int)v;
+  for (u64 m = __ballot(hub); m; m &= m - 1) {
+    const int l = __builtin_ctzll(m);
+    const int a = __shfl(g0, l), b = __shfl(g1, l), hv = __shfl((int)v, l);
+    for (int g = a + lane_id(); g < b; g += kWave) gsrc[g] = hv;
   }
 }
 
@@ -711,17 +720,25 @@ __global__ __launch_bounds__(kInitWaves * kWave) __attribute__((amdgpu_waves_per
 __global__ __launch_bounds__(64) void k_pick_light(int edges, const int* __restrict__ eweight, int T,
                                                    const int* __restrict__ dthr, int* __restrict__ out)
 {
-  if (dthr) T = dthr[0];
   const int lane = lane_id();
-  int c = 0;
+  if (edges <= 0) {
+    if (lane == 0) *out = 0;
+    return;
+  }
+  // (every sampled position is below edges, so the loads need no branch)
+  int wv[16];
+#pragma unroll
   for (int k = 0; k < 16; k++) {
     const long long i = k * kWave + lane;
-    const long long p = i * edges / 1024 + edges / 2048;
-    c += p < edges && eweight[p] <= T;
+    wv[k] = eweight[i * edges / 1024 + edges / 2048];
   }
+  if (dthr) T = dthr[0];
+  int c = 0;
+#pragma unroll
+  for (int k = 0; k < 16; k++) c += wv[k] <= T;
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off);
-  if (lane == 0) *out = edges > 0 && 10 * c >= 1024;
+  if (lane == 0) *out = 10 * c >= 1024;
 }
 
 // Skip flags of the sparse heavy initialisation: task g (positions
