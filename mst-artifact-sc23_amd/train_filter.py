@@ -2,14 +2,14 @@
 solves (needs a HIP device).
 
 1. Generates N seeded graphs: 2D grids, random geometric graphs and RMAT,
-   2^12..2^20 vertices (log-uniform), mixed degrees and weight ranges, and
-   optionally --large more of 2^21..2^22.5 vertices for the single-solve
-   network (48 of them helped geometric and Erdos-Renyi graphs far above
-   2^20 vertices but moved the 512x512 grid from the unfiltered path to a
-   slower filtered one, so the shipped network is trained without them).
-   The union network also sees Erdos-Renyi graphs (--union-er of them):
-   the batched workload mixes them in, and a network that never saw one
-   chose q near 0.5 where about 0.2 is best.
+   2^12..2^20 vertices (log-uniform), mixed degrees and weight ranges;
+   --union-er Erdos-Renyi graphs of the same sizes (the batched workload
+   mixes them in, and a network that never saw one chose q near 0.5 where
+   about 0.2 is best); and --large grids, geometric and RMAT graphs of
+   --large-range vertices.  The network extrapolates the trend of its
+   labels in log2(n) linearly: trained on 2^12..2^20 only it chose q = 0.22
+   on the geometric graph of 2^24 vertices, whose measured optimum is 0.5,
+   so the shipped network also sees 2^21..2^24.
 2. For each graph: device features and weight samples, then one solve per
    candidate quantile q (threshold = sorted sample of rank floor(q*1023);
    the last candidate is the unfiltered path).  Each candidate's cost is
@@ -17,18 +17,29 @@ solves (needs a HIP device).
        cost = launch_us * rounds + init_bytes / bw_init + round_bytes / bw_round
    where init_bytes counts the setup and initialisation passes and their
    records, and round_bytes 24 B per record entering a round and 96 B per
-   survivor.  The label is log2 of the cheapest q.  Two cost models give
+   survivor.  The label is log2 of the cheapest q; the unfiltered path is
+   labelled OFF_LABEL, so that the midpoint between it and q = 0.5 lies at
+   the device's cut (q >= 0.9 selects the unfiltered path) and a prediction
+   between the two modes resolves to the nearer one.  Two cost models give
    two label sets:
-     single  LAUNCH_US per round, every byte at BW: one graph per solve,
-             so every round's fixed cost is the graph's own;
-     union   LAUNCH_US / UNION_SHARE per round: in a batched arena about
-             UNION_SHARE graphs share every round, so a graph's marginal
-             cost is its bytes; round bytes are taken at UNION_ROUND_BW
-             (slower rates, 0.3-2 TB/s, left the labels almost unchanged
-             and did not make the batch faster).
+     single  SINGLE_LAUNCH_US per round, initialisation bytes at
+             SINGLE_BW_INIT, round bytes at SINGLE_BW_ROUND: one graph per
+             solve, so every round's fixed cost is the graph's own.  The
+             constants come from device-timed solves of 18 graphs (grids,
+             geometric, RMAT and Erdos-Renyi, 2^16..2^26 vertices) at every
+             candidate: they minimise the measured time lost at the model's
+             choice (geomean 0.8% over the per-graph optimum, worst 10%;
+             8 us per round with 3 TB/s lost 3.3%, worst 28%, choosing
+             filtered paths where one round costs more than 8 us);
+     union   UNION_LAUNCH_US per round: in a batched arena about a thousand
+             graphs share every round, so a graph's marginal cost is its
+             bytes, taken at BW (slower rates, 0.3-2 TB/s, left the labels
+             almost unchanged and did not make the batch faster).
 3. Full-batch Adam on the device (emst_mlp_train), one network per cost
-   model, then an evaluation on held-out graphs against the plain rule
-   (ECL_MST.cu:432-443) under the same cost model.
+   model (the single network on every row; the union network on the
+   first --graphs rows and the Erdos-Renyi rows), then an evaluation on
+   held-out graphs against the plain rule (ECL_MST.cu:432-443) under the
+   same cost model.
 Writes <out>/filter_mlp.f32, <out>/filter_mlp_union.f32 (fp32 parameters),
 <out>/filter_costs.npz (features and the per-candidate rounds and bytes of
 every training graph) and <out>/filter_mlp.json.
@@ -46,10 +57,12 @@ sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 import emst  # noqa: E402
 
 QS = [2.0 ** (-k / 2.0) for k in range(14, 0, -1)] + [1.0]   # 2^-7 .. 2^-0.5, then unfiltered
-LAUNCH_US = 8.0
-UNION_SHARE = 1000.0
+SINGLE_LAUNCH_US = 32.0
+SINGLE_BW_INIT = 4.0e12
+SINGLE_BW_ROUND = 3.0e12
+UNION_LAUNCH_US = 8.0 / 1000.0
 BW = 3.0e12
-UNION_ROUND_BW = 3.0e12
+OFF_LABEL = 0.7
 
 
 class CostModel:
@@ -63,8 +76,8 @@ class CostModel:
         return {"launch_us": self.launch_us, "bw_init": self.bw_init, "bw_round": self.bw_round}
 
 
-MODELS = {"single": CostModel(LAUNCH_US, BW, BW),
-          "union": CostModel(LAUNCH_US / UNION_SHARE, BW, UNION_ROUND_BW)}
+MODELS = {"single": CostModel(SINGLE_LAUNCH_US, SINGLE_BW_INIT, SINGLE_BW_ROUND),
+          "union": CostModel(UNION_LAUNCH_US, BW, BW)}
 
 
 def make_graph(i, seed, kind=None, logn_range=(12, 20)):
@@ -131,6 +144,16 @@ def best_q(rows, model):
     return QS[int(np.argmin(costs))]
 
 
+def label_of(qs, name):
+    """Regression targets: log2 q, with the unfiltered path at OFF_LABEL
+    for the single network.  The union network's targets stay log2 q
+    (unfiltered = 0), the targets its shipped file was trained on."""
+    y = np.log2(qs)
+    if name == "single":
+        y = np.where(qs >= 1.0, OFF_LABEL, y)
+    return y.astype(np.float32)
+
+
 def plain_cost(solver, g, model):
     uf, T = emst.plain_threshold(g)
     st = solver.solve(emst.FILTER_PLAIN if uf else emst.FILTER_OFF, T)
@@ -161,6 +184,111 @@ def evaluate(solver, params, model, holdout, seed, er_every=0):
     return ratio, ev
 
 
+# Graphs of the cost-model calibration (--calibrate): name, generator.
+CALIB_GRAPHS = [
+    ("grid 256x256", lambda: emst.gen_grid(256, 256, 24, 1, 1 << 20)),
+    ("grid 512x512", lambda: emst.gen_grid(512, 512, 1, 1, 1 << 20)),
+    ("grid 1024x1024", lambda: emst.gen_grid(1024, 1024, 14, 1, 1 << 20)),
+    ("grid 2048x2048 w<2^16", lambda: emst.gen_grid(2048, 2048, 13, 1, 1 << 16)),
+    ("grid 4096x4096", lambda: emst.gen_grid(4096, 4096, 12, 1, 1 << 20)),
+    ("rgg 2^20 deg 6", lambda: emst.gen_rgg(1 << 20, 6.0, 21, 1 << 24)),
+    ("rgg 2^22 deg 6 w<2^20", lambda: emst.gen_rgg(1 << 22, 6.0, 7, 1 << 20)),
+    ("rgg 2^23 deg 4 w<2^16", lambda: emst.gen_rgg(1 << 23, 4.0, 8, 1 << 16)),
+    ("rgg 2^23 deg 8", lambda: emst.gen_rgg(1 << 23, 8.0, 9, 1 << 24)),
+    ("rgg 2^24 deg 6", lambda: emst.gen_rgg(1 << 24, 6.0, 3, 1 << 24)),
+    ("er 2^20 deg 8", lambda: emst.gen_er(1 << 20, 8.0, 23, 1, 1 << 20)),
+    ("er 2^22 deg 10", lambda: emst.gen_er(1 << 22, 10.0, 10, 1, 1 << 20)),
+    ("rmat 20 ef 16", lambda: emst.gen_rmat(20, 16, 0.57, 0.19, 0.19, 22, 1, 1 << 24)),
+    ("rmat 22 ef 4 w<2^10", lambda: emst.gen_rmat(22, 4, 0.55, 0.2, 0.2, 17, 1, 1 << 10)),
+    ("rmat 22 ef 16 w<2^20", lambda: emst.gen_rmat(22, 16, 0.57, 0.19, 0.19, 15, 1, 1 << 20)),
+    ("rmat 23 ef 8 (0.5,0.2,0.2)", lambda: emst.gen_rmat(23, 8, 0.5, 0.2, 0.2, 16, 1, 1 << 24)),
+    ("rmat 24 ef 16", lambda: emst.gen_rmat(24, 16, 0.57, 0.19, 0.19, 2, 1, 1 << 24)),
+    ("rmat 26 ef 16 w<=64", lambda: emst.gen_rmat(26, 16, 0.57, 0.19, 0.19, 5, 1, 65)),
+]
+
+
+def calibrate(path, graphs, reps=3):
+    """Device-timed solves of the calibration graphs at every candidate
+    (median of reps, default configuration), with the reference schedule's
+    round and record counts of each candidate (parallel records kept, the
+    schedule the labels price); the plain rule and the shipped network are
+    timed beside them.  Then the single cost model's constants are searched
+    for the smallest measured time lost at the model's choice.  Writes the
+    measurements to path (JSON) and prints a table."""
+    solver = emst.Solver(0)
+    params = np.fromfile(os.path.join(os.path.dirname(os.path.abspath(__file__)), "filter_mlp.f32"), np.float32)
+    out = {}
+    for name, mk in CALIB_GRAPHS:
+        if graphs and name not in graphs:
+            continue
+        t0 = time.time()
+        g = mk()
+        solver.set_config(flags=0)
+        solver.set_filter_model(params)
+        solver.load(g)
+        _, _, _, feat, samples = solver.learned_threshold()
+        upw = np.sort(g.eweight[g.upper_mask()])
+
+        def timed(mode, thr):
+            solver.solve(mode, thr)
+            sts = [solver.solve(mode, thr) for _ in range(reps)]
+            return float(np.median([x.ms for x in sts])), sts[-1]
+
+        cands = [(emst.FILTER_OFF, emst.INT_MAX) if q >= 1.0 else
+                 (emst.FILTER_PLAIN, int(samples[int(math.floor(q * 1023))])) for q in QS]
+        rows = [{"q": q, "T": thr, "ms": timed(mode, thr)[0]} for q, (mode, thr) in zip(QS, cands)]
+        uf, pt = emst.plain_threshold(g)
+        plain_ms, _ = timed(emst.FILTER_PLAIN if uf else emst.FILTER_OFF, pt)
+        learned_ms, lst = timed(emst.FILTER_LEARNED, 0)
+        solver.set_config(flags=emst.F_NO_PAIR_DEDUP)
+        for r, (mode, thr) in zip(rows, cands):
+            st = solver.solve(mode, thr)
+            hu = int(len(upw) - np.searchsorted(upw, thr, side="right")) if mode == emst.FILTER_PLAIN else 0
+            r["rounds"] = st.light_rounds + st.heavy_rounds
+            r["init_bytes"], r["round_bytes"] = schedule_bytes(g, st, hu)
+        out[name] = {"n": g.nodes, "m": g.edges, "feat": [float(x) for x in feat], "rows": rows,
+                     "plain_ms": plain_ms, "learned_ms": learned_ms, "learned_q": float(lst.learned_q)}
+        best = min(rows, key=lambda r: r["ms"])
+        print(f"[calibrate] {name}: n={g.nodes} m={g.edges} best q={best['q']:.3f} {best['ms']:.3f} ms, "
+              f"plain {plain_ms:.3f} ms, shipped network q={lst.learned_q:.3f} {learned_ms:.3f} ms "
+              f"({time.time() - t0:.0f} s)", flush=True)
+        print("    " + " ".join(f"{r['q']:.3f}:{r['ms']:.3f}" for r in rows), flush=True)
+        with open(path, "w") as f:
+            json.dump(out, f)
+        del g
+    solver.close()
+    fit_constants(out)
+
+
+def model_regret(calib, launch_us, bw_init, bw_round):
+    """Measured time at the model's choice over the best candidate's, per
+    graph, for the single-solve cost model with these constants."""
+    m = CostModel(launch_us, bw_init, bw_round)
+    reg = {}
+    for name, g in calib.items():
+        costs = [m.cost(r["rounds"], r["init_bytes"], r["round_bytes"]) for r in g["rows"]]
+        ms = [r["ms"] for r in g["rows"]]
+        reg[name] = ms[int(np.argmin(costs))] / min(ms)
+    return reg
+
+
+def fit_constants(calib):
+    grid = [(lu, bi, br) for lu in (4, 8, 12, 16, 20, 24, 32, 40, 48, 64)
+            for bi in (1e12, 2e12, 3e12, 4e12, 6e12) for br in (1e12, 2e12, 3e12, 4e12, 6e12)]
+    scored = []
+    for c in grid:
+        reg = model_regret(calib, *c)
+        scored.append((float(np.exp(np.mean(np.log(list(reg.values()))))), max(reg.values()), c))
+    scored.sort(key=lambda x: (round(x[0], 6), x[1]))
+    print("[calibrate] cost-model constants by measured regret (geomean, worst):")
+    for gm, mx, c in scored[:8]:
+        print(f"    launch {c[0]:>4} us  init {c[1] / 1e12:.0f} TB/s  rounds {c[2] / 1e12:.0f} TB/s: {gm:.4f} {mx:.3f}")
+    for label, c in (("shipped", (SINGLE_LAUNCH_US, SINGLE_BW_INIT, SINGLE_BW_ROUND)), ("8 us, 3 TB/s", (8.0, 3e12, 3e12))):
+        reg = model_regret(calib, *c)
+        print(f"[calibrate] {label}: geomean {np.exp(np.mean(np.log(list(reg.values())))):.4f}, worst "
+              f"{max(reg.values()):.3f} ({max(reg, key=reg.get)})")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--graphs", type=int, default=2000)
@@ -171,9 +299,17 @@ def main():
     ap.add_argument("--out", default="filter_train", help="directory for the trained networks and labels")
     ap.add_argument("--costs", default=None, help="reuse a filter_costs.npz instead of solving the graphs again")
     ap.add_argument("--union-er", type=int, default=667, help="Erdos-Renyi graphs added for the union network")
-    ap.add_argument("--large", type=int, default=0,
-                    help="graphs of 2^21-2^22.5 vertices added for the single-solve network")
+    ap.add_argument("--large", type=int, default=240,
+                    help="graphs of --large-range vertices added for the single-solve network")
+    ap.add_argument("--large-range", default="21,24",
+                    help="log2 vertex range of the --large graphs")
+    ap.add_argument("--calibrate", default=None, metavar="JSON",
+                    help="time every candidate on the calibration graphs, write JSON, fit the cost-model constants")
+    ap.add_argument("--calibrate-graphs", default="", help="comma list of calibration graph names (default: all)")
     a = ap.parse_args()
+    if a.calibrate:
+        calibrate(a.calibrate, [x for x in a.calibrate_graphs.split(",") if x])
+        return
     os.makedirs(a.out, exist_ok=True)
     solver = emst.Solver(0)
     # Labels price the reference schedule's worklists: parallel-record
@@ -191,7 +327,7 @@ def main():
     total = a.graphs + a.union_er + a.large
     for i in range(len(X), total):
         if i >= a.graphs + a.union_er:
-            kind, g = make_graph(i, a.seed, None, (21, 22.5))
+            kind, g = make_graph(i, a.seed, None, tuple(float(x) for x in a.large_range.split(",")))
         else:
             kind, g = make_graph(i, a.seed, 3 if i >= a.graphs else None)
         feat, rows = label_graph(solver, g)
@@ -200,7 +336,7 @@ def main():
         BI.append([bi for _, bi, _ in rows])
         BR.append([br for _, _, br in rows])
         kinds.append(kind)
-        if i % 50 == 0:
+        if i % 50 == 0 or i >= a.graphs + a.union_er:
             print(f"[train_filter] labelled {i}/{total} ({time.time()-t0:.0f}s) last {kind} n={g.nodes}", flush=True)
     X = np.array(X[:total], np.float32)
     R = np.array(R[:total], np.int64)
@@ -211,18 +347,17 @@ def main():
              kinds=np.array(kinds), qs=np.array(QS))
     info = {"params": emst.MLP_PARAMS, "graphs": int(len(X)), "seed": a.seed, "epochs": a.epochs, "lr": a.lr,
             "candidates": QS, "models": {}}
-    # The single network trains on the grid, geometric and RMAT rows (the
-    # first --graphs and the --large ones); the union network on the first
-    # --graphs and the Erdos-Renyi rows.
+    # The single network trains on every row; the union network on the
+    # first --graphs and the Erdos-Renyi rows.
     base = list(range(a.graphs))
     er_rows = list(range(a.graphs, a.graphs + a.union_er))
     large_rows = list(range(a.graphs + a.union_er, total))
-    for name, fname, rows, er_every in (("single", "filter_mlp.f32", base + large_rows, 0),
+    for name, fname, rows, er_every in (("single", "filter_mlp.f32", base + er_rows + large_rows, 4 if a.union_er else 0),
                                         ("union", "filter_mlp_union.f32", base + er_rows, 4 if a.union_er else 0)):
         model = MODELS[name]
         rows_used = len(rows)
         qs = [best_q(list(zip(R[i], BI[i], BR[i])), model) for i in rows]
-        Y = np.log2(np.array(qs, np.float32))
+        Y = label_of(np.array(qs, np.float64), name)
         p0 = emst.init_params(a.seed)
         t1 = time.time()
         params, loss = emst.mlp_train(p0, X[rows], Y, a.epochs, a.lr, a.seed)

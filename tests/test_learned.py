@@ -133,9 +133,9 @@ def test_trained_model_beats_plain_rule(which):
     s.set_filter_model(np.fromfile(path, np.float32))
     ratios = []
     for i in range(24):
-        # The union network also trained on Erdos-Renyi graphs: every fourth
+        # Both networks also trained on Erdos-Renyi graphs: every fourth
         # held-out graph is one.
-        er = which == "union" and i % 4 == 3
+        er = i % 4 == 3
         kind, g = TF.make_graph(20_000_000 + i, 99, 3 if er else None)
         s.load(g)
         pc, pst = TF.plain_cost(s, g, model)
