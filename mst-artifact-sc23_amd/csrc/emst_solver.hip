@@ -255,7 +255,7 @@ __device__ __forceinline__ void init_finish(Stage& st, int4* seg, int& s_fill, i
   }
 }
 
-template <bool HEAVY, int SUB>
+template <bool HEAVY, int SUB, bool UNION>
 __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
     const int* __restrict__ nindex, const int* __restrict__ nlist, const int* __restrict__ eweight,
     const int* __restrict__ gsrc, int nodes, int edges, int T, const int* __restrict__ label,
@@ -323,15 +323,57 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
       for (int u = 0; u < SUB; u++) {
         const int j = (int)(gbeg + u * kWave + lane);
         const bool up = nl[u] > src[u];
-        const int t = (gthr && up) ? gthr[vgid[src[u]]] : T;
+        const int t = (UNION && up) ? gthr[vgid[src[u]]] : T;
         const bool keep = up && w[u] <= t;
         stage_push(st, keep, make_int4(src[u], nl[u], w[u], j), seg, &s_fill);
       }
+    } else if (!UNION) {
+      // One threshold: every light edge (w <= T) lies inside one component
+      // after the light phase, so only the upper-direction positions with
+      // w > T (the reference's own test, ECL_MST.cu:197) need roots.  Their
+      // weights are loaded with the source classes (positions of a task are
+      // consecutive, so the loads are coalesced), and the class and root
+      // gathers are skipped for the light ones.  The class map resolves
+      // most roots without touching parent: a member of the marked
+      // component has root bigroot, a root is its own root; only the
+      // remaining endpoints (inside other non-trivial components) are
+      // looked up.
+      bool hv[SUB];
+      int a[SUB], b[SUB], wt[SUB];
+      unsigned cv[SUB];
+#pragma unroll
+      for (int u = 0; u < SUB; u++) {
+        const long long p = gbeg + u * kWave + lane;
+        const bool up = nl[u] > src[u];
+        wt[u] = up ? eweight[p] : 0;
+        cv[u] = up ? (vclass[(unsigned)src[u] >> 4] >> ((src[u] & 15) * 2)) & 3u : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < SUB; u++) {
+        hv[u] = nl[u] > src[u] && wt[u] > T;
+        a[u] = bigroot;
+        b[u] = bigroot;
+        if (hv[u]) {
+          const unsigned cu = (vclass[(unsigned)nl[u] >> 4] >> ((nl[u] & 15) * 2)) & 3u;
+          a[u] = (cv[u] & 1u) ? bigroot : (cv[u] & 2u) ? src[u] : -1;
+          b[u] = (cu & 1u) ? bigroot : (cu & 2u) ? nl[u] : -1;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < SUB; u++) {
+        if (a[u] < 0) a[u] = label[src[u]];
+        if (b[u] < 0) b[u] = label[nl[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < SUB; u++) {
+        const long long p = gbeg + u * kWave + lane;
+        const bool keep = hv[u] && a[u] != b[u];
+        stage_push(st, keep, make_int4(a[u], b[u], wt[u], (int)p), seg, &s_fill);
+      }
     } else {
-      // The class map resolves most roots without touching parent: a
-      // member of the marked component has root bigroot, a root is its own
-      // root; only the remaining endpoints (inside other non-trivial
-      // components) are looked up.
+      // A threshold per graph of a union: the crossing test alone ("roots
+      // differ" implies w > T after the light phase), weights read only
+      // for crossing edges.
       bool up[SUB];
       int a[SUB], b[SUB];
 #pragma unroll
@@ -1353,16 +1395,23 @@ typedef void (*SelFn)(const int4*, const int*, int4*, int*, const int*, u64*, in
 typedef void (*JoinFn)(int4*, int*, int4*, int*, int*, u64*, uint8_t*, int, int*, long long*, int, const u64*, u64*,
                        ListView, const int*);
 
-InitFn init_fn(bool heavy, int sub)
+// uni: a union of graphs with a threshold per graph (gthr[vgid[v]])
+template <bool UNION>
+InitFn init_fn_t(bool heavy, int sub)
 {
   if (heavy) {
-    if (sub == 4) return k_init_wl<true, 4>;
-    if (sub == 16) return k_init_wl<true, 16>;
-    return k_init_wl<true, 8>;
+    if (sub == 4) return k_init_wl<true, 4, UNION>;
+    if (sub == 16) return k_init_wl<true, 16, UNION>;
+    return k_init_wl<true, 8, UNION>;
   }
-  if (sub == 4) return k_init_wl<false, 4>;
-  if (sub == 16) return k_init_wl<false, 16>;
-  return k_init_wl<false, 8>;
+  if (sub == 4) return k_init_wl<false, 4, UNION>;
+  if (sub == 16) return k_init_wl<false, 16, UNION>;
+  return k_init_wl<false, 8, UNION>;
+}
+
+InitFn init_fn(bool heavy, int sub, bool uni)
+{
+  return uni ? init_fn_t<true>(heavy, sub) : init_fn_t<false>(heavy, sub);
 }
 
 SparseFn sparse_fn()
@@ -1698,10 +1747,10 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
                            rix->mirrored ? 1 : 0, hasmem, wb.minv, wl[0], z.seg_cap, wb.segcnt, &d_cnt[0], stage_cap,
                            d_stats, dthr, &d_cnt[2]);
       }
-      const InitFn f = init_fn(phase == 1, sub);
+      const InitFn f = init_fn(phase == 1, sub, vgid != nullptr);
       hipLaunchKernelGGL(f, dim3(z.init_grid), dim3(kInitWaves * kWave), init_lds, s, d_nindex, d_nlist, d_eweight,
                          wb.gsrc, nodes, edges, T, wb.parent, wb.vclass, wl[0], z.seg_cap, wb.segcnt, &d_cnt[0],
-                         stage_cap, d_stats, dthr, phase == 0 ? vgid : nullptr, phase == 0 ? gthr : nullptr,
+                         stage_cap, d_stats, dthr, vgid, phase == 0 ? gthr : nullptr,
                          &d_cnt[2], phase == 1 ? (sparse ? &d_cnt[3] : nullptr) : (light_offers ? &d_cnt[4] : nullptr));
       if (!fixed) HIPCHK(hipEventRecord(ctx->ev[3 + 2 * phase], s));
       HIPCHK(hipGetLastError());
@@ -1851,9 +1900,9 @@ int emst_create(int device, emst_ctx** out)
   if (ok) {
     const int dyn = (int)ctx->lds_per_block;
     for (int sub : {4, 8, 16}) {
-      for (int h = 0; h < 2 && ok; h++)
-        ok = ok && hipFuncSetAttribute((const void*)init_fn(h == 1, sub), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       dyn) == hipSuccess;
+      for (int h = 0; h < 4 && ok; h++)
+        ok = ok && hipFuncSetAttribute((const void*)init_fn(h & 1, sub, h >= 2),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, dyn) == hipSuccess;
     }
     ok = ok && hipFuncSetAttribute((const void*)sparse_fn(), hipFuncAttributeMaxDynamicSharedMemorySize, dyn) ==
                    hipSuccess;
