@@ -190,11 +190,11 @@ def run_single(a, emst):
         heavy_bytes += 32 * stl.heavy
     if stl.light_offers:
         light_bytes += 32 * stl.light
-    light_name = "k_init_light_offers<4>" if stl.light_offers else "k_init_wl<false, 8>"
+    light_name = "k_init_light_offers<4>" if stl.light_offers else "k_init_wl<false, 8, false>"
     heavy_ms = statistics.mean(x.init_heavy_ms for x in st_l)
     light_ms = statistics.mean(x.init_light_ms for x in st_l)
     tr = read_traffic(f"traffic_{a.workload}.json") if a.scale is None else None
-    heavy_name = "k_init_heavy_sparse<4>" if stl.sparse_heavy else "k_init_wl<true, 8>"
+    heavy_name = "k_init_heavy_sparse<4>" if stl.sparse_heavy else "k_init_wl<true, 8, false>"
     kernels = {"light": {"kernel": f"{light_name} (light worklist initialisation)", "avg_launch_ms": light_ms,
                          "alg_bytes_per_launch": light_bytes,
                          "achieved": light_bytes / (light_ms * 1e-3) / 1e9,

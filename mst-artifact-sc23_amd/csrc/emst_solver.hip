@@ -912,8 +912,9 @@ __device__ __forceinline__ unsigned spread16(unsigned x)
 // parent[v] = root(v), and the vertex class map: two bits per vertex,
 // sixteen vertices per word, bit 0 = (root(v) == *big), bit 1 = (v is a
 // root).  Concurrent writers only shorten chains toward the same root, so
-// readers stay correct.  With hasmem (cleared by k_setup), every root
-// outside the marked component that has other members gets hasmem[root] = 1.
+// readers stay correct.  With hasmem (cleared by k_setup) and the sparse
+// heavy pass chosen (big[1]), its only reader, every root outside the marked
+// component that has other members gets hasmem[root] = 1.
 __global__ __launch_bounds__(256) void k_flatten(int nodes, int* __restrict__ parent, int* __restrict__ cnt,
                                                  const int* __restrict__ big, unsigned* __restrict__ vclass,
                                                  const int* __restrict__ dthr, unsigned char* __restrict__ hasmem)
@@ -924,12 +925,14 @@ __global__ __launch_bounds__(256) void k_flatten(int nodes, int* __restrict__ pa
     cnt[1] = 0;
   }
   if (dthr && dthr[1] == 0) return;  // unfiltered: no heavy phase
-  const int bigroot = *big;
+  const int bigroot = big[0];
+  // member flags only for the sparse heavy pass (big[1], set by k_pick_big)
+  unsigned char* const hm = big[1] ? hasmem : nullptr;
   int r = -1;
   if (v < nodes) {
     r = find_root((int)v, parent);
     parent[v] = r;
-    if (hasmem && r != (int)v && r != bigroot) hasmem[r] = 1;
+    if (hm && r != (int)v && r != bigroot) hm[r] = 1;
   }
   const u64 mb = __ballot(r == bigroot && r >= 0);
   const u64 mr = __ballot(r >= 0 && r == (int)v);
