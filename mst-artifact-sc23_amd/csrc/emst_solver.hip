@@ -316,6 +316,18 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
         if (!HEAVY) w_n[u] = p < edges ? ld_stream(&eweight[p]) : 0;
       }
     }
+    // A union's graphs are packed by vertex id, so a task whose first and
+    // last sources lie in one graph has that graph's threshold (scalar
+    // loads).  In a task that spans two graphs the light pass gathers it per
+    // position, and the heavy pass tests every upper-direction position for
+    // crossing (a light edge never crosses, so only the gathers are lost).
+    int tT = T;
+    bool span = false;
+    if (UNION) {
+      const int g0 = vgid[__builtin_amdgcn_readfirstlane(base)];
+      span = g0 != vgid[__builtin_amdgcn_readfirstlane(vend)];
+      tT = HEAVY && span ? INT_MIN : gthr[g0];
+    }
     int src[SUB];
     task_sources<SUB>(src, tb, gbeg, base, edges, nodes, nindex, marks, lane);
     if (!HEAVY) {
@@ -323,14 +335,15 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
       for (int u = 0; u < SUB; u++) {
         const int j = (int)(gbeg + u * kWave + lane);
         const bool up = nl[u] > src[u];
-        const int t = (UNION && up) ? gthr[vgid[src[u]]] : T;
+        const int t = (UNION && span && up) ? gthr[vgid[src[u]]] : tT;
         const bool keep = up && w[u] <= t;
         stage_push(st, keep, make_int4(src[u], nl[u], w[u], j), seg, &s_fill);
       }
-    } else if (!UNION) {
-      // One threshold: every light edge (w <= T) lies inside one component
-      // after the light phase, so only the upper-direction positions with
-      // w > T (the reference's own test, ECL_MST.cu:197) need roots.  Their
+    } else {
+      // Every light edge (w <= T; a union's graph's own T) lies inside one
+      // component after the light phase, so only the upper-direction
+      // positions with w > T (the reference's own test, ECL_MST.cu:197)
+      // need roots.  Their
       // weights are loaded with the source classes (positions of a task are
       // consecutive, so the loads are coalesced), and the class and root
       // gathers are skipped for the light ones.  The class map resolves
@@ -350,7 +363,7 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
       }
 #pragma unroll
       for (int u = 0; u < SUB; u++) {
-        hv[u] = nl[u] > src[u] && wt[u] > T;
+        hv[u] = nl[u] > src[u] && wt[u] > tT;
         a[u] = bigroot;
         b[u] = bigroot;
         if (hv[u]) {
@@ -369,36 +382,6 @@ __global__ __launch_bounds__(kInitWaves * kWave) void k_init_wl(
         const long long p = gbeg + u * kWave + lane;
         const bool keep = hv[u] && a[u] != b[u];
         stage_push(st, keep, make_int4(a[u], b[u], wt[u], (int)p), seg, &s_fill);
-      }
-    } else {
-      // A threshold per graph of a union: the crossing test alone ("roots
-      // differ" implies w > T after the light phase), weights read only
-      // for crossing edges.
-      bool up[SUB];
-      int a[SUB], b[SUB];
-#pragma unroll
-      for (int u = 0; u < SUB; u++) {
-        up[u] = nl[u] > src[u];
-        a[u] = bigroot;
-        b[u] = bigroot;
-        if (up[u]) {
-          const unsigned cv = (vclass[(unsigned)src[u] >> 4] >> ((src[u] & 15) * 2)) & 3u;
-          const unsigned cu = (vclass[(unsigned)nl[u] >> 4] >> ((nl[u] & 15) * 2)) & 3u;
-          a[u] = (cv & 1u) ? bigroot : (cv & 2u) ? src[u] : -1;
-          b[u] = (cu & 1u) ? bigroot : (cu & 2u) ? nl[u] : -1;
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < SUB; u++) {
-        if (a[u] < 0) a[u] = label[src[u]];
-        if (b[u] < 0) b[u] = label[nl[u]];
-      }
-#pragma unroll
-      for (int u = 0; u < SUB; u++) {
-        const long long p = gbeg + u * kWave + lane;
-        const bool keep = up[u] && a[u] != b[u];
-        const int wt = keep ? eweight[p] : 0;
-        stage_push(st, keep, make_int4(a[u], b[u], wt, (int)p), seg, &s_fill);
       }
     }
     base = base_n;
@@ -1671,6 +1654,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
   if (init_lds > ctx->lds_per_block) return ctx_fail(ctx, EMST_E_ARG, "stage_recs exceeds the LDS of a workgroup");
   const int nphases = (vt ? vt->any_filter : use_filter != EMST_FILTER_OFF) ? 2 : 1;
   const int* vgid = vt ? vt->vgid : nullptr;
+  if (vt && (!vt->vgid || !vt->gthr)) return ctx_fail(ctx, EMST_E_ARG, "a union needs its vertex-graph map and thresholds");
   const int* gthr = vt ? vt->gthr : nullptr;
   int* d_cnt = wb.cnt;
   long long* d_stats = wb.stats;
@@ -1753,7 +1737,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       const InitFn f = init_fn(phase == 1, sub, vgid != nullptr);
       hipLaunchKernelGGL(f, dim3(z.init_grid), dim3(kInitWaves * kWave), init_lds, s, d_nindex, d_nlist, d_eweight,
                          wb.gsrc, nodes, edges, T, wb.parent, wb.vclass, wl[0], z.seg_cap, wb.segcnt, &d_cnt[0],
-                         stage_cap, d_stats, dthr, vgid, phase == 0 ? gthr : nullptr,
+                         stage_cap, d_stats, dthr, vgid, gthr,
                          &d_cnt[2], phase == 1 ? (sparse ? &d_cnt[3] : nullptr) : (light_offers ? &d_cnt[4] : nullptr));
       if (!fixed) HIPCHK(hipEventRecord(ctx->ev[3 + 2 * phase], s));
       HIPCHK(hipGetLastError());
