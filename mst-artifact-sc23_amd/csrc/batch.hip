@@ -44,6 +44,8 @@ struct Chunk {
   int* d_gflag = nullptr;          // per-graph filter flag
   long long* d_voff = nullptr;     // per-graph vertex offsets (chunk-local), g1-g0+1 entries
   long long* d_eoff = nullptr;     // per-graph edge offsets
+  int* d_rev = nullptr;            // reverse positions (sorted, symmetric lists), else null
+  bool rev_mirrored = false;
   std::vector<long long> voff, eoff;
   // captured schedule
   hipGraphExec_t exec = nullptr;
@@ -75,7 +77,7 @@ namespace {
 void free_chunk(Chunk& c)
 {
   for (void* p : {(void*)c.d_nindex, (void*)c.d_nlist, (void*)c.d_eweight, (void*)c.d_inMST, (void*)c.d_vgid,
-                  (void*)c.d_gthr, (void*)c.d_gflag, (void*)c.d_voff, (void*)c.d_eoff})
+                  (void*)c.d_gthr, (void*)c.d_gflag, (void*)c.d_voff, (void*)c.d_eoff, (void*)c.d_rev})
     if (p) (void)hipFree(p);
   if (c.exec) (void)hipGraphExecDestroy(c.exec);
   c = Chunk();
@@ -153,17 +155,21 @@ int emst_batch_create(emst_ctx* ctx, int32_t ngraphs, const emst_graph* graphs, 
     }
     std::vector<int32_t> nindex((size_t)c.nodes + 1), nlist((size_t)std::max(c.edges, 1)),
         ew((size_t)std::max(c.edges, 1)), vgid((size_t)c.nodes);
-    std::vector<long long> ups(G, 0);
+    std::vector<long long> ups(G, 0), lows(G, 0);
     emst::parallel_for(G, [&](int64_t lo, int64_t hi, int) {
       for (int64_t k = lo; k < hi; k++) {
         const emst_graph& gg = graphs[c.g0 + k];
         const long long vo = c.voff[k], eo = c.eoff[k];
-        long long up = 0;
+        long long up = 0, low = 0;
         for (int32_t v = 0; v < gg.nodes; v++) {
           nindex[vo + v] = (int32_t)(eo + gg.nindex[v]);
           vgid[vo + v] = (int32_t)k;
-          for (int32_t j = gg.nindex[v]; j < gg.nindex[v + 1]; j++) up += gg.nlist[j] > v;
+          for (int32_t j = gg.nindex[v]; j < gg.nindex[v + 1]; j++) {
+            up += gg.nlist[j] > v;
+            low += gg.nlist[j] < v;
+          }
         }
+        lows[k] = low;
         for (int32_t j = 0; j < gg.edges; j++) {
           nlist[eo + j] = (int32_t)(vo + gg.nlist[j]);
           ew[eo + j] = gg.eweight[j];
@@ -173,7 +179,9 @@ int emst_batch_create(emst_ctx* ctx, int32_t ngraphs, const emst_graph* graphs, 
       }
     });
     nindex[c.nodes] = c.edges;
+    long long m_low = 0;
     for (long long u : ups) c.m_up += u;
+    for (long long l : lows) m_low += l;
     const size_t ne = (size_t)std::max(c.edges, 1);
     bool ok = hipMalloc((void**)&c.d_nindex, ((size_t)c.nodes + 1) * 4) == hipSuccess &&
               hipMalloc((void**)&c.d_nlist, ne * 4) == hipSuccess && hipMalloc((void**)&c.d_eweight, ne * 4) == hipSuccess &&
@@ -197,7 +205,26 @@ int emst_batch_create(emst_ctx* ctx, int32_t ngraphs, const emst_graph* graphs, 
          hipMemcpyAsync(c.d_voff, c.voff.data(), (size_t)(G + 1) * 8, hipMemcpyHostToDevice, cs) == hipSuccess &&
          hipMemcpyAsync(c.d_eoff, c.eoff.data(), (size_t)(G + 1) * 8, hipMemcpyHostToDevice, cs) == hipSuccess &&
          hipStreamSynchronize(cs) == hipSuccess;
-    if (!ok) rc = ctx_fail(ctx, EMST_E_HIP, "batch arena upload failed");
+    if (!ok) {
+      rc = ctx_fail(ctx, EMST_E_HIP, "batch arena upload failed");
+      break;
+    }
+    // reverse positions (sorted, symmetric lists with one weight per edge):
+    // the light initialisation then also makes the first offers
+    if (c.edges > 0 && c.m_up == m_low) {
+      if (hipMalloc((void**)&c.d_rev, (size_t)c.edges * 4) != hipSuccess) {
+        rc = ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of an arena's reverse positions failed");
+        break;
+      }
+      bool rok = false, mir = false;
+      rc = build_rev(ctx, c.nodes, c.edges, c.d_nindex, c.d_nlist, c.d_eweight, c.d_rev, cs, &rok, &mir);
+      if (rc) break;
+      if (!rok || !mir) {
+        (void)hipFree(c.d_rev);
+        c.d_rev = nullptr;
+      }
+      c.rev_mirrored = mir;
+    }
   }
   if (!rc) {
     // size the shared work buffers for the largest arena up front, so that a
@@ -359,8 +386,14 @@ int emst_batch_solve(emst_batch* b, int use_filter, const int32_t* thresholds, i
       (void)hipGraphExecDestroy(c.exec);
       c.exec = nullptr;
     }
+    RevIndex rix;
+    if (c.d_rev) {
+      rix.rev = c.d_rev;
+      rix.mirrored = c.rev_mirrored;
+    }
     int rc = run_solve(ctx, c.nodes, c.edges, c.d_nindex, c.d_nlist, c.d_eweight,
-                       any ? EMST_FILTER_PLAIN : EMST_FILTER_OFF, INT_MAX, c.d_inMST, s, c.m_up, &vt, &cs);
+                       any ? EMST_FILTER_PLAIN : EMST_FILTER_OFF, INT_MAX, c.d_inMST, s, c.m_up, &vt, &cs, nullptr,
+                       nullptr, &rix);
     if (rc) return rc;
     total_ms += cs.ms;
     b->last_arena_ms[ci] = cs.ms;
@@ -379,7 +412,7 @@ int emst_batch_solve(emst_batch* b, int use_filter, const int32_t* thresholds, i
       if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return ctx_fail(ctx, EMST_E_HIP, "hipStreamBeginCapture failed");
       rc = run_solve(ctx, c.nodes, c.edges, c.d_nindex, c.d_nlist, c.d_eweight, any ? EMST_FILTER_PLAIN : EMST_FILTER_OFF,
-                     INT_MAX, c.d_inMST, s, c.m_up, &vt, nullptr, &fs);
+                     INT_MAX, c.d_inMST, s, c.m_up, &vt, nullptr, &fs, nullptr, &rix);
       const hipError_t ec = hipStreamEndCapture(s, &graph);
       if (rc || ec != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);

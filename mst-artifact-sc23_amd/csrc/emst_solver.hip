@@ -634,12 +634,13 @@ __global__ __launch_bounds__(kInitWaves * kWave) __attribute__((amdgpu_waves_per
 // processed.
 // Runs when *decide is set (k_pick_light); k_init_wl<false> runs otherwise.
 // ---------------------------------------------------------------------------
-template <int SUB>
+template <int SUB, bool UNION>
 __global__ __launch_bounds__(kInitWaves * kWave) __attribute__((amdgpu_waves_per_eu(5))) void k_init_light_offers(
     const int* __restrict__ nindex, const int* __restrict__ nlist, const int* __restrict__ eweight,
     const int* __restrict__ gsrc, int nodes, int edges, int T, const int* __restrict__ rev, u64* __restrict__ minv,
     int4* __restrict__ wl, long long seg_cap, int* __restrict__ segcnt, int* __restrict__ cnt, int stage_cap,
-    long long* __restrict__ stats, const int* __restrict__ dthr, const int* __restrict__ decide)
+    long long* __restrict__ stats, const int* __restrict__ dthr, const int* __restrict__ decide,
+    const int* __restrict__ vgid, const int* __restrict__ gthr)
 {
   constexpr int TE = SUB * kWave;
   static_assert(TE == kStartEdges, "light offer tasks are start-table blocks");
@@ -680,6 +681,15 @@ __global__ __launch_bounds__(kInitWaves * kWave) __attribute__((amdgpu_waves_per
     const long long gn = g + nwaves;
     int nl_n[SUB], w_n[SUB], rv_n[SUB], base_n = 0, vend_n = 0;
     if (gn < ngroups) load_task(gn, nl_n, w_n, rv_n, base_n, vend_n);
+    // a union's threshold: the task's graph's (scalar loads), per position
+    // in a task that spans two graphs (its graphs are packed by vertex id)
+    int tT = T;
+    bool span = false;
+    if (UNION) {
+      const int g0 = vgid[__builtin_amdgcn_readfirstlane(base)];
+      span = g0 != vgid[__builtin_amdgcn_readfirstlane(vend)];
+      tT = gthr[g0];
+    }
     int src[SUB];
     task_sources<SUB>(src, tb, gbeg, base, edges, nodes, nindex, marks, lane);
     const int pbeg = (int)gbeg;  // positions fit in int (edges < 2^31)
@@ -690,7 +700,8 @@ __global__ __launch_bounds__(kInitWaves * kWave) __attribute__((amdgpu_waves_per
     for (int u = 0; u < SUB; u++) {
       const int p = pbeg + u * kWave + lane;
       lower[u] = nl[u] < src[u];
-      light[u] = p < edges && nl[u] != src[u] && w[u] <= T;
+      const int t = (UNION && span && p < edges) ? gthr[vgid[src[u]]] : tT;
+      light[u] = p < edges && nl[u] != src[u] && w[u] <= t;
       j[u] = (light[u] && lower[u]) ? rv[u] : p;
     }
     // each source's minimum over this task: LDS slots by vertex (vertex
@@ -743,11 +754,11 @@ __global__ __launch_bounds__(kInitWaves * kWave) __attribute__((amdgpu_waves_per
 // pays when light edges are a large share of the adjacency.  *out = 1 when
 // at least a tenth of 1024 evenly spaced positions are light (w <= T).
 __global__ __launch_bounds__(64) void k_pick_light(int edges, const int* __restrict__ eweight, int T,
-                                                   const int* __restrict__ dthr, int* __restrict__ out)
+                                                   const int* __restrict__ dthr, int* __restrict__ out, int force)
 {
   const int lane = lane_id();
-  if (edges <= 0) {
-    if (lane == 0) *out = 0;
+  if (edges <= 0 || force) {
+    if (lane == 0) *out = edges > 0;
     return;
   }
   // (every sampled position is below edges, so the loads need no branch)
@@ -1375,7 +1386,7 @@ typedef void (*SparseFn)(const int*, const int*, const int*, const int*, int, in
                          const unsigned char*, const int*, int, const unsigned char*, u64*, int4*, long long, int*,
                          int*, int, long long*, const int*, const int*);
 typedef void (*LightFn)(const int*, const int*, const int*, const int*, int, int, int, const int*, u64*, int4*,
-                        long long, int*, int*, int, long long*, const int*, const int*);
+                        long long, int*, int*, int, long long*, const int*, const int*, const int*, const int*);
 typedef void (*SelFn)(const int4*, const int*, int4*, int*, const int*, u64*, int, int, int, long long*, int, u64*,
                       const u64*, ListView, const int*);
 typedef void (*JoinFn)(int4*, int*, int4*, int*, int*, u64*, uint8_t*, int, int*, long long*, int, const u64*, u64*,
@@ -1405,9 +1416,10 @@ SparseFn sparse_fn()
   return k_init_heavy_sparse<kSparseSub>;
 }
 
-LightFn light_offers_fn()
+// uni: a union of graphs with a threshold per graph
+LightFn light_offers_fn(bool uni)
 {
-  return k_init_light_offers<kSparseSub>;
+  return uni ? k_init_light_offers<kSparseSub, true> : k_init_light_offers<kSparseSub, false>;
 }
 
 SelFn sel_fn(bool first, int waves)
@@ -1625,13 +1637,13 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
   }
   // the sparse heavy initialisation needs reverse positions, skip flags and
   // the member flags of the roots
-  const bool sparse = rix && rix->rev && wb.skip && wb.hasmem && !(cfg.flags & EMST_F_NO_SPARSE_HEAVY) &&
+  const bool sparse = rix && rix->rev && !vt && wb.skip && wb.hasmem && !(cfg.flags & EMST_F_NO_SPARSE_HEAVY) &&
                       sparse_lds_bytes(std::max(64, cfg.stage_recs)) <= ctx->lds_per_block;
   unsigned char* const hasmem = sparse ? wb.hasmem : nullptr;
   // the light initialisation that also makes the first offers needs every
   // vertex's list to hold all its edges with their upper copies' weights;
   // with it both light passes are enqueued and k_pick_light picks one
-  const bool light_offers = rix && rix->rev && rix->mirrored && !vt && !(cfg.flags & EMST_F_NO_LIGHT_OFFERS) &&
+  const bool light_offers = rix && rix->rev && rix->mirrored && !(cfg.flags & EMST_F_NO_LIGHT_OFFERS) &&
                             sparse_lds_bytes(std::max(64, cfg.stage_recs)) <= ctx->lds_per_block;
   const WorkSizes z = work_sizes(ctx->cus, cfg, nodes, edges, wl_cap);
   const bool learned = use_filter == EMST_FILTER_LEARNED && !vt;
@@ -1722,10 +1734,13 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       // and the other return at once.  The sparse pass also makes the first
       // round's offers, and then the offer-only select returns at once too.
       if (phase == 0 && light_offers) {
-        hipLaunchKernelGGL(k_pick_light, dim3(1), dim3(kWave), 0, s, edges, d_eweight, T, dthr, &d_cnt[4]);
-        hipLaunchKernelGGL(light_offers_fn(), dim3(z.init_grid), dim3(kInitWaves * kWave), sparse_lds_bytes(stage_cap),
-                           s, d_nindex, d_nlist, d_eweight, wb.gsrc, nodes, edges, T, rix->rev, wb.minv, wl[0],
-                           z.seg_cap, wb.segcnt, &d_cnt[0], stage_cap, d_stats, dthr, &d_cnt[4]);
+        // a union always takes the offers pass (its light share is per graph)
+        hipLaunchKernelGGL(k_pick_light, dim3(1), dim3(kWave), 0, s, edges, d_eweight, T, dthr, &d_cnt[4],
+                           vgid != nullptr ? 1 : 0);
+        hipLaunchKernelGGL(light_offers_fn(vgid != nullptr), dim3(z.init_grid), dim3(kInitWaves * kWave),
+                           sparse_lds_bytes(stage_cap), s, d_nindex, d_nlist, d_eweight, wb.gsrc, nodes, edges, T,
+                           rix->rev, wb.minv, wl[0], z.seg_cap, wb.segcnt, &d_cnt[0], stage_cap, d_stats, dthr,
+                           &d_cnt[4], vgid, gthr);
       }
       if (phase == 1 && sparse) {
         const SparseFn f = sparse_fn();
@@ -1893,7 +1908,9 @@ int emst_create(int device, emst_ctx** out)
     }
     ok = ok && hipFuncSetAttribute((const void*)sparse_fn(), hipFuncAttributeMaxDynamicSharedMemorySize, dyn) ==
                    hipSuccess;
-    ok = ok && hipFuncSetAttribute((const void*)light_offers_fn(), hipFuncAttributeMaxDynamicSharedMemorySize,
+    ok = ok && hipFuncSetAttribute((const void*)light_offers_fn(true), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   dyn) == hipSuccess;
+    ok = ok && hipFuncSetAttribute((const void*)light_offers_fn(false), hipFuncAttributeMaxDynamicSharedMemorySize,
                                    dyn) == hipSuccess;
   }
   if (!ok) {
