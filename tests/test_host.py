@@ -485,3 +485,38 @@ def test_cli_rejects_bad_variant_before_the_device(tmp_path):
         if bad == "":
             continue  # an empty name selects the default schedule (needs the device)
         assert r.returncode == 255 and "bad EMST_VARIANT" in r.stderr, (bad, r.stderr)
+
+
+def test_threshold_cost_model_calibration_on_synthetic_times():
+    """The cost-model search of train_filter.py --calibrate: on measurements
+    generated from a known cost model (rounds at 30 us, bytes at 4 and 3
+    TB/s), a setting equal to the generating one loses nothing, and a model
+    that undercharges rounds picks a wrong candidate and loses time."""
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                    "mst-artifact-sc23_amd"))
+    import train_filter as TF
+    rng = np.random.default_rng(5)
+    truth = TF.CostModel(30.0, 4e12, 3e12)
+    calib = {}
+    for g in range(12):
+        rows = []
+        for q in TF.QS:
+            rounds = int(rng.integers(4, 30))
+            bi, br = float(rng.uniform(1e8, 4e9)), float(rng.uniform(1e8, 6e9))
+            rows.append({"q": q, "rounds": rounds, "init_bytes": bi, "round_bytes": br,
+                         "ms": truth.cost(rounds, bi, br) / 1e3})
+        calib[f"g{g}"] = {"rows": rows}
+    reg = TF.model_regret(calib, 30.0, 4e12, 3e12)
+    assert all(abs(v - 1.0) < 1e-12 for v in reg.values())
+    # rounds nearly free: the model follows bytes alone and picks worse candidates
+    reg_bad = TF.model_regret(calib, 0.01, 4e12, 3e12)
+    assert max(reg_bad.values()) > 1.0
+    # the label rule picks the cheapest candidate under the model
+    rows = calib["g0"]["rows"]
+    q = TF.best_q([(r["rounds"], r["init_bytes"], r["round_bytes"]) for r in rows], truth)
+    assert q == min(rows, key=lambda r: r["ms"])["q"]
+    # the unfiltered label sits above log2 0.5 by as much as the cut q >= 0.9 sits below 0
+    y = TF.label_of(np.array([0.5, 1.0, 0.25]), "single")
+    assert y[1] == np.float32(TF.OFF_LABEL) and y[0] == -1.0 and y[2] == -2.0
+    assert abs((y[0] + y[1]) / 2 - np.log2(0.9)) < 0.01
+    assert TF.label_of(np.array([1.0]), "union")[0] == 0.0
