@@ -78,6 +78,7 @@ struct emst_ctx {
   // ECL_MST.cu:432-441); d_thr and d_q hold it while learned_cached is set
   bool learned_cached = false;
   int learned_filter = 0;
+  int learned_T = INT_MAX;  // the cached decision's threshold (a solve then runs as a plain one at it)
   // incremented whenever a work buffer is (re)allocated: captured launch
   // schedules hold raw buffer addresses and are valid for one generation
   unsigned long long buf_gen = 0;

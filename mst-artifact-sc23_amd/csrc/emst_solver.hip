@@ -1647,8 +1647,13 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
                             sparse_lds_bytes(std::max(64, cfg.stage_recs)) <= ctx->lds_per_block;
   const WorkSizes z = work_sizes(ctx->cus, cfg, nodes, edges, wl_cap);
   const bool learned = use_filter == EMST_FILTER_LEARNED && !vt;
-  const int T = use_filter == EMST_FILTER_PLAIN ? threshold : INT_MAX;
-  const int* dthr = learned ? ctx->d_thr : nullptr;
+  // a learned threshold decided before the solve (cached per loaded graph)
+  // runs as a plain solve at that threshold: the kernels read no device
+  // decision
+  const bool known = learned && learned_cached;
+  const int T = use_filter == EMST_FILTER_PLAIN ? threshold
+                                                : (known && ctx->learned_filter ? ctx->learned_T : INT_MAX);
+  const int* dthr = learned && !known ? ctx->d_thr : nullptr;
   // Host-sync ablation (the reference reads the worklist size back after
   // every kernel1, ECL_MST.cu:318): the host waits for every round's
   // posted size before it enqueues the next round, and no round runs in the
@@ -1664,7 +1669,8 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
   const int stage_cap = std::max(64, cfg.stage_recs);
   const size_t init_lds = init_lds_bytes(stage_cap, task_edges);
   if (init_lds > ctx->lds_per_block) return ctx_fail(ctx, EMST_E_ARG, "stage_recs exceeds the LDS of a workgroup");
-  const int nphases = (vt ? vt->any_filter : use_filter != EMST_FILTER_OFF) ? 2 : 1;
+  const int nphases =
+      (vt ? vt->any_filter : known ? ctx->learned_filter != 0 : use_filter != EMST_FILTER_OFF) ? 2 : 1;
   const int* vgid = vt ? vt->vgid : nullptr;
   if (vt && (!vt->vgid || !vt->gthr)) return ctx_fail(ctx, EMST_E_ARG, "a union needs its vertex-graph map and thresholds");
   const int* gthr = vt ? vt->gthr : nullptr;
@@ -2082,6 +2088,7 @@ int emst_solve(emst_ctx* ctx, int use_filter, int32_t threshold, emst_stats* sta
     HIPCHK(hipMemcpyAsync(thr, ctx->d_thr, 8, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->learned_filter = thr[1];
+    ctx->learned_T = thr[0];
     ctx->learned_cached = true;
   }
   return run_solve(ctx, ctx->n, ctx->m, ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, use_filter, threshold,
