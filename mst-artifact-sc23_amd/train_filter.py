@@ -5,8 +5,8 @@ solves (needs a HIP device).
    2^12..2^20 vertices (log-uniform), mixed degrees and weight ranges;
    --union-er Erdos-Renyi graphs of the same sizes (the batched workload
    mixes them in, and a network that never saw one chose q near 0.5 where
-   about 0.2 is best); and --large grids, geometric and RMAT graphs of
-   --large-range vertices.  The network extrapolates the trend of its
+   about 0.2 is best); and --large grids, geometric and RMAT graphs and
+   --large-er Erdos-Renyi graphs of --large-range vertices.  The network extrapolates the trend of its
    labels in log2(n) linearly: trained on 2^12..2^20 only it chose q = 0.22
    on the geometric graph of 2^24 vertices, whose measured optimum is 0.5,
    so the shipped network also sees 2^21..2^24.
@@ -303,6 +303,8 @@ def main():
                     help="graphs of --large-range vertices added for the single-solve network")
     ap.add_argument("--large-range", default="21,24",
                     help="log2 vertex range of the --large graphs")
+    ap.add_argument("--large-er", type=int, default=60,
+                    help="Erdos-Renyi graphs of --large-range vertices added for the single-solve network")
     ap.add_argument("--calibrate", default=None, metavar="JSON",
                     help="time every candidate on the calibration graphs, write JSON, fit the cost-model constants")
     ap.add_argument("--calibrate-graphs", default="", help="comma list of calibration graph names (default: all)")
@@ -324,10 +326,13 @@ def main():
         z = np.load(a.costs)
         X, R, BI, BR = (list(z[k]) for k in ("X", "rounds", "init_bytes", "round_bytes"))
         kinds = [str(k) for k in z["kinds"]]
-    total = a.graphs + a.union_er + a.large
+    total = a.graphs + a.union_er + a.large + a.large_er
+    large_range = tuple(float(x) for x in a.large_range.split(","))
     for i in range(len(X), total):
-        if i >= a.graphs + a.union_er:
-            kind, g = make_graph(i, a.seed, None, tuple(float(x) for x in a.large_range.split(",")))
+        if i >= a.graphs + a.union_er + a.large:
+            kind, g = make_graph(i, a.seed, 3, large_range)
+        elif i >= a.graphs + a.union_er:
+            kind, g = make_graph(i, a.seed, None, large_range)
         else:
             kind, g = make_graph(i, a.seed, 3 if i >= a.graphs else None)
         feat, rows = label_graph(solver, g)
