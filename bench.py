@@ -284,6 +284,8 @@ def run_single(a, emst):
             "post_filter_edges": {"learned": stl.heavy, "plain": st_p[-1].heavy}},
         "config": {"workload": w["desc"] if a.scale is None else w["desc"].replace(str(w.get("scale")), str(a.scale), 1),
                    "nodes": n, "edges": m, "learned_q": stl.learned_q, "threshold": stl.threshold_used,
+                   "threshold_decided": "once per loaded graph, before the timed solves, for both rules "
+                                        "(the reference picks its threshold before gpuMST, ECL_MST.cu:432-441)",
                    "filter": bool(stl.filter_used), "light": stl.light, "heavy": stl.heavy,
                    "rounds": [stl.light_rounds, stl.heavy_rounds], "tail_rounds": stl.tail_rounds,
                    "solver_config": s.config().as_dict()},
