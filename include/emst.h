@@ -88,6 +88,7 @@ typedef struct {
   float learned_q;           /* quantile predicted by the network, -1 if not used      */
   int32_t sparse_heavy;      /* 1 when the sparse heavy initialisation ran             */
   int32_t light_offers;      /* 1 when the light initialisation made the first offers  */
+  int32_t relabelled;        /* 1 when the solve ran on the degree-ordered copy         */
 } emst_stats;
 
 /* Tunables.  Zero in a field selects the built-in default. */
@@ -128,6 +129,18 @@ typedef struct {
  * sees all its edges in its own list), and that round's select is skipped.
  * Results, records and round counts are the same either way (ablation). */
 #define EMST_F_NO_LIGHT_OFFERS 4096
+/* Keep the caller's vertex numbering.  By default emst_load renumbers a
+ * graph with skewed degrees (largest degree at least 32 times the mean) and
+ * sorted, symmetric lists by decreasing degree and solves on a copy of its
+ * CSR in that numbering (rebuilt on the device, once per load): the hubs'
+ * per-vertex entries then share cache lines.  Keys, records and inMST keep
+ * the caller's edge positions, so results and round counts are the same
+ * either way; only the root ids that emst_fetch_parent reports differ.
+ * Read at emst_load (ablation). */
+#define EMST_F_NO_RELABEL 8192
+/* Renumber at emst_load whenever the lists are sorted and symmetric,
+ * whatever the degrees (testing). */
+#define EMST_F_RELABEL 16384
 
 /* The de-optimization ladder (reference: Deoptimizations/ECL_MST_No*.cu,
  * removing one optimization per file).  EMST_F_LADDER runs the reference's

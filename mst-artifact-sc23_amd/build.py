@@ -31,7 +31,7 @@ CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-Wno-unused-fun
 
 HOST_SOURCES = ["egr_io.cpp", "generators.cpp", "cpu_mst.cpp", "formats.cpp"]
 HIP_SOURCES = ["emst_solver.hip", "learned.hip", "batch.hip", "ladder.hip", "contract.hip", "edgelist.hip",
-               "noalloc.hip"]
+               "noalloc.hip", "relabel.hip"]
 CLI_SOURCES = ["ecl_mst_cli.cpp", "ecl_gen_cli.cpp", "ecl_convert_cli.cpp", "contract_mst_cli.cpp", "edgelist_mst_cli.cpp"]
 
 

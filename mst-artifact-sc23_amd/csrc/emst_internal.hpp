@@ -9,6 +9,18 @@
 #include "../../include/emst.h"
 #include "emst_kernels.hpp"
 
+struct Relabel {
+  bool ok = false;
+  int* nindex = nullptr;
+  int* nlist = nullptr;
+  int* eweight = nullptr;
+  int* kid = nullptr;
+  int* order = nullptr;
+  int* newid = nullptr;
+  size_t cap_nodes = 0, cap_edges = 0;
+  double ms = 0;  // build time at the last load
+};
+
 struct emst_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -48,6 +60,12 @@ struct emst_ctx {
   size_t cap_rev = 0;
   bool rev_ok = false;
   bool rev_mirrored = false;
+  // degree-ordered layout of the loaded graph (emst_load, skewed degrees):
+  // vertices renumbered by decreasing degree, lists sorted by the new ids,
+  // each position's key id (its edge's upper-direction position in the
+  // caller's CSR) and that copy's weight; order maps new ids to the
+  // caller's, newid the reverse
+  Relabel rl;
   int* d_cnt = nullptr;
   long long* d_stats = nullptr;
   int* h_flag = nullptr;
@@ -66,6 +84,7 @@ struct emst_ctx {
   hipEvent_t ev[8] = {};
   int32_t last_nodes = 0;
   const int* last_parent = nullptr;   // parent array of the last solve (context or scratch)
+  bool last_relabelled = false;       // ... in the degree-ordered numbering
   // learned threshold
   unsigned short* d_mlp16 = nullptr;  // bf16 parameters
   int have_model = 0;
@@ -111,6 +130,11 @@ struct WorkBufs {
 struct RevIndex {
   const int* rev = nullptr;
   bool mirrored = false;  // every edge's two copies carry the same weight
+  // rev holds the key id of every position (a relabelled layout): the
+  // fused passes key records and offers by rev[p] in both directions; the
+  // streaming passes write positions, which the phase's first select
+  // replaces by their key ids
+  bool kid = false;
 };
 
 // Element counts of the work buffers for a solve of the given size under a
@@ -165,6 +189,15 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
 int build_rev(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
               const int* d_eweight, int* rev, hipStream_t s, bool* ok, bool* mirrored);
 
+// Builds the context's degree-ordered layout (ctx->rl) of the loaded graph
+// from its device copy and reverse positions (relabel.hip); sets rl.ok.
+int build_relabel(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
+                  const int* d_eweight, const int* rev, hipStream_t s);
+void free_relabel(Relabel& rl);
+// Copies a parent array of the degree-ordered layout to host memory in the
+// caller's numbering.
+int relabel_parent(emst_ctx* ctx, const int* parent, int32_t nodes, int32_t* out, hipStream_t s);
+
 // The reference's kernel sequence with switchable optimizations (the
 // EMST_F_LADDER and EMST_F_NO_* bits of the context's flags): one graph, a
 // given threshold or none.
@@ -181,6 +214,7 @@ int noalloc_setup(int device);
 constexpr int kLadderFlags = EMST_F_LADDER | EMST_F_NO_WARP | EMST_F_NO_IMPLICIT_PC | EMST_F_NO_ONE_DIRECTION |
                              EMST_F_NO_TUPLES | EMST_F_NO_DATA_DRIVEN | EMST_F_NO_EDGE_BASED;
 constexpr int kAllFlags = EMST_F_HOST_SYNC | EMST_F_NO_GUARD | EMST_F_NO_COMBINE | EMST_F_NO_PAIR_DEDUP |
-                          EMST_F_NO_SPARSE_HEAVY | EMST_F_NO_LIGHT_OFFERS | kLadderFlags;
+                          EMST_F_NO_SPARSE_HEAVY | EMST_F_NO_LIGHT_OFFERS | EMST_F_NO_RELABEL | EMST_F_RELABEL |
+                          kLadderFlags;
 
 }  // namespace emst

@@ -446,7 +446,7 @@ __global__ __launch_bounds__(kInitWaves * kWave) __attribute__((amdgpu_waves_per
     const int* __restrict__ nindex, const int* __restrict__ nlist, const int* __restrict__ eweight,
     const int* __restrict__ gsrc, int nodes, int edges, const int* __restrict__ label,
     const unsigned* __restrict__ vclass, const unsigned char* __restrict__ skip, const int* __restrict__ rev,
-    int mirrored, const unsigned char* __restrict__ hasmem, u64* __restrict__ minv, int4* __restrict__ wl,
+    int mirrored, int kidmode, const unsigned char* __restrict__ hasmem, u64* __restrict__ minv, int4* __restrict__ wl,
     long long seg_cap, int* __restrict__ segcnt, int* __restrict__ cnt, int stage_cap, long long* __restrict__ stats,
     const int* __restrict__ dthr, const int* __restrict__ bigroot_p)
 {
@@ -545,7 +545,7 @@ __global__ __launch_bounds__(kInitWaves * kWave) __attribute__((amdgpu_waves_per
 #pragma unroll
     for (int u = 0; u < SUB; u++) {
       const int p = pbeg + u * kWave + lane;
-      j[u] = (cross[u] && lower[u]) ? rev[p] : p;
+      j[u] = (cross[u] && (lower[u] || kidmode)) ? rev[p] : p;
       wv[u] = (cross[u] && !mirrored) ? eweight[j[u]] : w[u];
     }
     // each source's minimum over this task: LDS slots by vertex (vertex
@@ -637,7 +637,8 @@ __global__ __launch_bounds__(kInitWaves * kWave) __attribute__((amdgpu_waves_per
 template <int SUB, bool UNION>
 __global__ __launch_bounds__(kInitWaves * kWave) __attribute__((amdgpu_waves_per_eu(5))) void k_init_light_offers(
     const int* __restrict__ nindex, const int* __restrict__ nlist, const int* __restrict__ eweight,
-    const int* __restrict__ gsrc, int nodes, int edges, int T, const int* __restrict__ rev, u64* __restrict__ minv,
+    const int* __restrict__ gsrc, int nodes, int edges, int T, const int* __restrict__ rev, int kidmode,
+    u64* __restrict__ minv,
     int4* __restrict__ wl, long long seg_cap, int* __restrict__ segcnt, int* __restrict__ cnt, int stage_cap,
     long long* __restrict__ stats, const int* __restrict__ dthr, const int* __restrict__ decide,
     const int* __restrict__ vgid, const int* __restrict__ gthr)
@@ -702,7 +703,7 @@ __global__ __launch_bounds__(kInitWaves * kWave) __attribute__((amdgpu_waves_per
       lower[u] = nl[u] < src[u];
       const int t = (UNION && span && p < edges) ? gthr[vgid[src[u]]] : tT;
       light[u] = p < edges && nl[u] != src[u] && w[u] <= t;
-      j[u] = (light[u] && lower[u]) ? rv[u] : p;
+      j[u] = (light[u] && (lower[u] || kidmode)) ? rv[u] : p;
     }
     // each source's minimum over this task: LDS slots by vertex (vertex
     // ranges up to kMarkSpan), else straight to minv
@@ -1078,7 +1079,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_select(
     const int4* __restrict__ wl_in, const int* __restrict__ cnt_in, int4* __restrict__ wl_out,
     int* __restrict__ cnt_out, const int* __restrict__ parent, u64* __restrict__ minv, int guard, int combine,
     int dedup, long long* __restrict__ stats, int rounds_slot, u64* __restrict__ cand, const u64* __restrict__ dead,
-    ListView lv, const int* __restrict__ fused)
+    ListView lv, const int* __restrict__ fused, const int* __restrict__ kid)
 {
   constexpr int BT = WAVES * kWave;
   constexpr int TILE = BT * kSelItems;
@@ -1120,6 +1121,16 @@ __global__ __launch_bounds__(WAVES * kWave) void k_select(
       // records joined by the first round are known to lie inside one tree
       live[k] = i < lim && !(dead && ((dead[(base + k * BT) / kWave + wid] >> lane) & 1ull));
       r[k] = live[k] ? wl_in[i] : make_int4(0, 0, 0, 0);
+    }
+    if (FIRST && kid) {
+      // a relabelled layout's initialisation wrote positions: the records
+      // take their key ids here (the list is joined in place)
+#pragma unroll
+      for (int k = 0; k < kSelItems; k++)
+        if (live[k]) r[k].w = kid[r[k].w];
+#pragma unroll
+      for (int k = 0; k < kSelItems; k++)
+        if (live[k]) reinterpret_cast<int*>(const_cast<int4*>(wl_in) + base + k * BT + tid)[3] = r[k].w;
     }
     bool keep[kSelItems];
     int mine = 0;
@@ -1383,12 +1394,12 @@ typedef void (*InitFn)(const int*, const int*, const int*, const int*, int, int,
                        const unsigned*, int4*, long long, int*, int*, int, long long*, const int*, const int*,
                        const int*, const int*, const int*);
 typedef void (*SparseFn)(const int*, const int*, const int*, const int*, int, int, const int*, const unsigned*,
-                         const unsigned char*, const int*, int, const unsigned char*, u64*, int4*, long long, int*,
-                         int*, int, long long*, const int*, const int*);
-typedef void (*LightFn)(const int*, const int*, const int*, const int*, int, int, int, const int*, u64*, int4*,
+                         const unsigned char*, const int*, int, int, const unsigned char*, u64*, int4*, long long,
+                         int*, int*, int, long long*, const int*, const int*);
+typedef void (*LightFn)(const int*, const int*, const int*, const int*, int, int, int, const int*, int, u64*, int4*,
                         long long, int*, int*, int, long long*, const int*, const int*, const int*, const int*);
 typedef void (*SelFn)(const int4*, const int*, int4*, int*, const int*, u64*, int, int, int, long long*, int, u64*,
-                      const u64*, ListView, const int*);
+                      const u64*, ListView, const int*, const int*);
 typedef void (*JoinFn)(int4*, int*, int4*, int*, int*, u64*, uint8_t*, int, int*, long long*, int, const u64*, u64*,
                        ListView, const int*);
 
@@ -1610,6 +1621,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
     if (rc == EMST_OK) {
       ctx->last_nodes = nodes;
       ctx->last_parent = ctx->d_parent;
+      ctx->last_relabelled = false;
     }
     return rc;
   }
@@ -1645,6 +1657,9 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
   // with it both light passes are enqueued and k_pick_light picks one
   const bool light_offers = rix && rix->rev && rix->mirrored && !(cfg.flags & EMST_F_NO_LIGHT_OFFERS) &&
                             sparse_lds_bytes(std::max(64, cfg.stage_recs)) <= ctx->lds_per_block;
+  // a relabelled layout keys every position through rev (its key ids)
+  const int kidmode = rix && rix->rev && rix->kid ? 1 : 0;
+  if (kidmode && (vt || !rix->mirrored)) return ctx_fail(ctx, EMST_E_ARG, "key ids need one graph with mirrored weights");
   const WorkSizes z = work_sizes(ctx->cus, cfg, nodes, edges, wl_cap);
   const bool learned = use_filter == EMST_FILTER_LEARNED && !vt;
   // a learned threshold decided before the solve (cached per loaded graph)
@@ -1679,6 +1694,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
   int4** wl = wb.wl;
   ctx->last_nodes = nodes;
   ctx->last_parent = wb.parent;
+  ctx->last_relabelled = kidmode != 0;
   const SelFn sel_first = sel_fn(true, waves), sel = sel_fn(false, waves);
   const JoinFn join = join_fn(waves);
   const ListView dense{nullptr, 0, 0};
@@ -1745,15 +1761,15 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
                            vgid != nullptr ? 1 : 0);
         hipLaunchKernelGGL(light_offers_fn(vgid != nullptr), dim3(z.init_grid), dim3(kInitWaves * kWave),
                            sparse_lds_bytes(stage_cap), s, d_nindex, d_nlist, d_eweight, wb.gsrc, nodes, edges, T,
-                           rix->rev, wb.minv, wl[0], z.seg_cap, wb.segcnt, &d_cnt[0], stage_cap, d_stats, dthr,
-                           &d_cnt[4], vgid, gthr);
+                           rix->rev, kidmode, wb.minv, wl[0], z.seg_cap, wb.segcnt, &d_cnt[0], stage_cap, d_stats,
+                           dthr, &d_cnt[4], vgid, gthr);
       }
       if (phase == 1 && sparse) {
         const SparseFn f = sparse_fn();
         hipLaunchKernelGGL(f, dim3(z.init_grid), dim3(kInitWaves * kWave), sparse_lds_bytes(stage_cap), s, d_nindex,
                            d_nlist, d_eweight, wb.gsrc, nodes, edges, wb.parent, wb.vclass, wb.skip, rix->rev,
-                           rix->mirrored ? 1 : 0, hasmem, wb.minv, wl[0], z.seg_cap, wb.segcnt, &d_cnt[0], stage_cap,
-                           d_stats, dthr, &d_cnt[2]);
+                           rix->mirrored ? 1 : 0, kidmode, hasmem, wb.minv, wl[0], z.seg_cap, wb.segcnt, &d_cnt[0],
+                           stage_cap, d_stats, dthr, &d_cnt[2]);
       }
       const InitFn f = init_fn(phase == 1, sub, vgid != nullptr);
       hipLaunchKernelGGL(f, dim3(z.init_grid), dim3(kInitWaves * kWave), init_lds, s, d_nindex, d_nlist, d_eweight,
@@ -1786,7 +1802,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       if (r == 0)
         hipLaunchKernelGGL(sel_first, dim3(grid), dim3(waves * kWave), 0, s, wl[0], &d_cnt[0], wl[1], &d_cnt[1],
                            wb.parent, wb.minv, guard, combine, dedup, d_stats, rounds_slot, wb.cand,
-                           (const u64*)nullptr, lv, fused);
+                           (const u64*)nullptr, lv, fused, kidmode ? rix->rev : nullptr);
       // the first join reads the candidate bitmasks; every join marks what it joined
       hipLaunchKernelGGL(join, dim3(grid), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt], &d_cnt[nxt],
                          wb.parent, wb.minv, d_inMST, tail_max, ctx->d_flag + slot, d_stats, rounds_slot,
@@ -1794,7 +1810,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
       // each select drops the records the join before it joined
       hipLaunchKernelGGL(sel, dim3(grid), dim3(waves * kWave), 0, s, wl[cur], &d_cnt[cur], wl[nxt], &d_cnt[nxt],
                          wb.parent, wb.minv, guard, combine, dedup, d_stats, rounds_slot, (u64*)nullptr,
-                         (const u64*)wb.cand, lv, (const int*)nullptr);
+                         (const u64*)wb.cand, lv, (const int*)nullptr, (const int*)nullptr);
       HIPCHK(hipGetLastError());
       slot++;
       if (fixed) {
@@ -1865,6 +1881,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
     stats->round_out_sum = hs[ST_OUT_SUM];
     stats->sparse_heavy = (int32_t)hs[ST_COUNT + 3];
     stats->light_offers = (int32_t)hs[ST_COUNT + 4];
+    stats->relabelled = kidmode;
   }
   return EMST_OK;
 }
@@ -1940,6 +1957,7 @@ void emst_destroy(emst_ctx* ctx)
                   ctx->d_lad_cnt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  free_relabel(ctx->rl);
   if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
   if (ctx->h_report) (void)hipHostFree(ctx->h_report);
   if (ctx->h_decide) (void)hipHostFree(ctx->h_decide);
@@ -2023,24 +2041,30 @@ int emst_load(emst_ctx* ctx, const emst_graph* g)
   // Lower-direction positions are counted too: reverse positions need as
   // many of them as upper-direction ones.
   const int nt = emst::host_threads();
-  std::vector<long long> part((size_t)nt, 0), part_lo((size_t)nt, 0);
+  std::vector<long long> part((size_t)nt, 0), part_lo((size_t)nt, 0), part_max((size_t)nt, 0);
   emst::parallel_for(g->nodes, [&](int64_t b, int64_t e, int t) {
-    long long c = 0, lo = 0;
-    for (int64_t v = b; v < e; v++)
+    long long c = 0, lo = 0, mx = 0;
+    for (int64_t v = b; v < e; v++) {
       for (int32_t j = g->nindex[v]; j < g->nindex[v + 1]; j++) {
         c += g->nlist[j] > v;
         lo += g->nlist[j] < v;
       }
+      mx = std::max<long long>(mx, g->nindex[v + 1] - g->nindex[v]);
+    }
     part[t] = c;
     part_lo[t] = lo;
+    part_max[t] = mx;
   }, nt);
-  long long up = 0, low = 0;
+  long long up = 0, low = 0, maxdeg = 0;
   for (int t = 0; t < nt; t++) {
     up += part[t];
     low += part_lo[t];
+    maxdeg = std::max(maxdeg, part_max[t]);
   }
   ctx->loaded = false;
   ctx->learned_cached = false;
+  ctx->last_parent = nullptr;  // buffers may move; a new graph has no solve yet
+  ctx->last_relabelled = false;
   ctx->rev_ok = false;
   ctx->rev_mirrored = false;
   if (up == low && ne > 0) {
@@ -2058,6 +2082,18 @@ int emst_load(emst_ctx* ctx, const emst_graph* g)
                         &ctx->rev_ok, &ctx->rev_mirrored);
     if (rc2) return rc2;
   }
+  // Skewed degrees (the largest at least 32 times the mean): solve on a copy
+  // renumbered by decreasing degree.  It needs the reverse positions, which
+  // give every lower-direction copy its key id.
+  ctx->rl.ok = false;
+  ctx->rl.ms = 0;
+  const int fl = ctx->cfg.flags;
+  const bool skewed = g->nodes >= 1024 && maxdeg * (long long)g->nodes >= 32LL * g->edges;
+  if (ctx->rev_ok && !(fl & EMST_F_NO_RELABEL) && ((fl & EMST_F_RELABEL) || skewed)) {
+    const int rc3 = build_relabel(ctx, g->nodes, g->edges, ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, ctx->d_rev,
+                                  ctx->stream);
+    if (rc3) return rc3;
+  }
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->n = g->nodes;
   ctx->m = g->edges;
@@ -2071,8 +2107,16 @@ int emst_solve(emst_ctx* ctx, int use_filter, int32_t threshold, emst_stats* sta
   if (!ctx) return EMST_E_ARG;
   if (!ctx->loaded) return ctx_fail(ctx, EMST_E_STATE, "emst_solve before emst_load");
   HIPCHK(hipSetDevice(ctx->device));
+  // the degree-ordered copy, unless the ladder (which keys by position) or
+  // the caller's numbering is asked for
+  const Relabel& rl = ctx->rl;
+  const bool use_rl = rl.ok && !(ctx->cfg.flags & (kLadderFlags | EMST_F_NO_RELABEL));
   RevIndex rix;
-  if (ctx->rev_ok) {
+  if (use_rl) {
+    rix.rev = rl.kid;
+    rix.mirrored = true;  // both copies carry the upper copy's weight
+    rix.kid = true;
+  } else if (ctx->rev_ok) {
     rix.rev = ctx->d_rev;
     rix.mirrored = ctx->rev_mirrored;
   }
@@ -2091,7 +2135,8 @@ int emst_solve(emst_ctx* ctx, int use_filter, int32_t threshold, emst_stats* sta
     ctx->learned_T = thr[0];
     ctx->learned_cached = true;
   }
-  return run_solve(ctx, ctx->n, ctx->m, ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, use_filter, threshold,
+  return run_solve(ctx, ctx->n, ctx->m, use_rl ? rl.nindex : ctx->d_nindex, use_rl ? rl.nlist : ctx->d_nlist,
+                   use_rl ? rl.eweight : ctx->d_eweight, use_filter, threshold,
                    ctx->d_inMST, ctx->stream, ctx->m_up, nullptr, stats, nullptr, nullptr, &rix,
                    use_filter == EMST_FILTER_LEARNED && ctx->learned_cached);
 }
@@ -2111,6 +2156,7 @@ int emst_fetch_parent(emst_ctx* ctx, int32_t* parent)
   if (!ctx || !parent) return EMST_E_ARG;
   if (!ctx->last_parent || ctx->last_nodes < 1) return ctx_fail(ctx, EMST_E_STATE, "no solve has run");
   HIPCHK(hipSetDevice(ctx->device));
+  if (ctx->last_relabelled) return relabel_parent(ctx, ctx->last_parent, ctx->last_nodes, parent, ctx->stream);
   HIPCHK(hipMemcpyAsync(parent, ctx->last_parent, (size_t)ctx->last_nodes * 4, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return EMST_OK;

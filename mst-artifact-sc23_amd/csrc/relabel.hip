@@ -1,0 +1,241 @@
+// relabel.hip -- the degree-ordered layout of a loaded graph.
+//
+// The solver's vertex-indexed arrays (parent, minv, the class map, the
+// label and member flags) are gathered at random by every round.  On a graph
+// with skewed degrees (RMAT) most records touch a few hundred thousand hub
+// vertices, but the caller's numbering scatters them over the whole id
+// range, so each gather fetches a line that holds one hot entry.  emst_load
+// therefore renumbers such a graph by decreasing degree (ties keep the
+// caller's order) and rebuilds its CSR in the new numbering with every list
+// sorted by the new ids.  The hubs' entries then share lines and stay in
+// L2; the hubs' lists also come first, so the largest light-phase component
+// (the hubs) owns a contiguous run of positions that the sparse heavy
+// initialisation skips whole.
+//
+// Results are unchanged.  The reference keys an edge by its weight and the
+// position j of its upper-direction copy (ECL_MST.cu:173-176, 222-223) and
+// marks inMST[j]; with distinct keys the forest of every phase is the
+// unique minimum forest under that order, whatever the vertex numbering.
+// The new layout keeps, per position, its edge's key id (j in the caller's
+// CSR: the position itself for an upper-direction copy, the reverse
+// position for a lower one) and that copy's weight, so records, offers and
+// inMST use the caller's keys and indices.  The passes that make the first
+// offers read the key ids with their task; the plain streaming passes emit
+// positions, and the phase's first select replaces each record's position by
+// its key id before any key is formed (a gather there overlaps the select's
+// own gathers; in the streaming pass it stalled the stream).  Both copies of an edge carry
+// its upper copy's weight, the one the reference reads.  Root ids are the
+// new numbering's: emst_fetch_parent maps them back.
+//
+// Built on the device from the uploaded CSR and its reverse positions: one
+// 64-bit radix sort of (new source, new neighbour) keys with the caller's
+// positions as values.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+
+#include "emst_internal.hpp"
+
+namespace {
+
+using emst::u64;
+
+// Degrees and vertex ids, the input of the degree sort.
+__global__ __launch_bounds__(256) void k_rl_deg(int n, const int* __restrict__ nindex, int* __restrict__ deg,
+                                                int* __restrict__ ids)
+{
+  const int v = blockIdx.x * 256 + threadIdx.x;
+  if (v < n) {
+    deg[v] = nindex[v + 1] - nindex[v];
+    ids[v] = v;
+  }
+}
+
+// newid is the inverse of order; ndeg[n] = 0 closes the scan to the new
+// offsets.
+__global__ __launch_bounds__(256) void k_rl_newid(int n, const int* __restrict__ order, int* __restrict__ newid,
+                                                  int* __restrict__ ndeg)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) newid[order[i]] = i;
+  if (i == 0) ndeg[n] = 0;
+}
+
+constexpr int kKeyPer = 1024;  // positions per workgroup of k_rl_keys
+
+// key[p] = newid[src(p)] << B | newid[nlist[p]], val[p] = p.  The workgroup
+// finds the source range of its positions once; each position then
+// searches only that range of offsets.
+__global__ __launch_bounds__(256) void k_rl_keys(int n, int m, int B, const int* __restrict__ nindex,
+                                                 const int* __restrict__ nlist, const int* __restrict__ newid,
+                                                 u64* __restrict__ key, int* __restrict__ val)
+{
+  __shared__ int s_lo, s_hi;
+  const long long p0 = (long long)blockIdx.x * kKeyPer;
+  // source of position p: the largest v in [lo, hi] with nindex[v] <= p
+  auto src_of = [&](long long p, int lo, int hi) {
+    while (lo < hi) {
+      const int mid = (int)(((long long)lo + hi + 1) >> 1);
+      if (nindex[mid] <= p) lo = mid;
+      else hi = mid - 1;
+    }
+    return lo;
+  };
+  if (threadIdx.x == 0) s_lo = src_of(p0, 0, n - 1);
+  if (threadIdx.x == 1) {
+    const long long p1 = p0 + kKeyPer < m ? p0 + kKeyPer - 1 : (long long)m - 1;
+    s_hi = src_of(p1, 0, n - 1);
+  }
+  __syncthreads();
+  const int lo = s_lo, hi = s_hi;
+  for (int k = threadIdx.x; k < kKeyPer; k += 256) {
+    const long long p = p0 + k;
+    if (p >= m) break;
+    const int v = src_of(p, lo, hi);
+    key[p] = ((u64)newid[v] << B) | (u64)(unsigned)newid[nlist[p]];
+    val[p] = (int)p;
+  }
+}
+
+// The new lists from the sorted keys: neighbour, key id and weight per new
+// position.
+__global__ __launch_bounds__(256) void k_rl_fill(int m, int B, const u64* __restrict__ key,
+                                                 const int* __restrict__ val, const int* __restrict__ order,
+                                                 const int* __restrict__ nlist, const int* __restrict__ eweight,
+                                                 const int* __restrict__ rev, int* __restrict__ nl2,
+                                                 int* __restrict__ w2, int* __restrict__ kid)
+{
+  const long long stride = (long long)gridDim.x * 256;
+  const u64 mask = (1ull << B) - 1;
+  for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < m; q += stride) {
+    const u64 k = key[q];
+    const int po = val[q];
+    const int vo = order[(int)(k >> B)];
+    const int yo = nlist[po];
+    const int j = yo < vo ? rev[po] : po;  // a self-loop keeps its own position (it is never a record)
+    nl2[q] = (int)(k & mask);
+    kid[q] = j;
+    w2[q] = eweight[j];
+  }
+}
+
+// A parent array in the new numbering, in the caller's: out[v] is the
+// caller's id of parent[newid[v]].
+__global__ __launch_bounds__(256) void k_rl_parent(int n, const int* __restrict__ parent,
+                                                   const int* __restrict__ newid, const int* __restrict__ order,
+                                                   int* __restrict__ out)
+{
+  const int v = blockIdx.x * 256 + threadIdx.x;
+  if (v < n) out[v] = order[parent[newid[v]]];
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf()
+  {
+    if (p) (void)hipFree(p);
+  }
+};
+
+}  // namespace
+
+namespace emst {
+
+void free_relabel(Relabel& rl)
+{
+  for (int* q : {rl.nindex, rl.nlist, rl.eweight, rl.kid, rl.order, rl.newid})
+    if (q) (void)hipFree(q);
+  rl = Relabel{};
+}
+
+int relabel_parent(emst_ctx* ctx, const int* parent, int32_t nodes, int32_t* out, hipStream_t s)
+{
+  const Relabel& rl = ctx->rl;
+  if (!rl.ok || (size_t)nodes > rl.cap_nodes) return ctx_fail(ctx, EMST_E_STATE, "no degree-ordered layout");
+  DevBuf tmp;
+  if (hipMalloc(&tmp.p, (size_t)std::max(nodes, 1) * 4)) return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc failed");
+  hipLaunchKernelGGL(k_rl_parent, dim3((nodes + 255) / 256), dim3(256), 0, s, nodes, parent, rl.newid, rl.order,
+                     (int*)tmp.p);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(out, tmp.p, (size_t)nodes * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return ctx_fail(ctx, EMST_E_HIP, "mapping the parent array back failed");
+  return EMST_OK;
+}
+
+int build_relabel(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
+                  const int* d_eweight, const int* rev, hipStream_t s)
+{
+  Relabel& rl = ctx->rl;
+  rl.ok = false;
+  if (nodes < 2 || edges < 1) return EMST_OK;
+  const auto t0 = std::chrono::steady_clock::now();
+  auto fail = [&](hipError_t e, const char* what) {
+    char b[256];
+    std::snprintf(b, sizeof(b), "HIP error %d in %s: %s", (int)e, what, hipGetErrorString(e));
+    return ctx_fail(ctx, EMST_E_HIP, b);
+  };
+#define RLCHK(call)                               \
+  do {                                            \
+    hipError_t e_ = (call);                       \
+    if (e_ != hipSuccess) return fail(e_, #call); \
+  } while (0)
+  const size_t n = (size_t)nodes, m = (size_t)edges;
+  if (n > rl.cap_nodes || m > rl.cap_edges || !rl.nindex) {
+    free_relabel(rl);
+    if (hipMalloc((void**)&rl.nindex, (n + 1) * 4) || hipMalloc((void**)&rl.order, n * 4) ||
+        hipMalloc((void**)&rl.newid, n * 4) || hipMalloc((void**)&rl.nlist, m * 4) ||
+        hipMalloc((void**)&rl.eweight, m * 4) || hipMalloc((void**)&rl.kid, m * 4)) {
+      free_relabel(rl);
+      return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of the degree-ordered layout failed");
+    }
+    rl.cap_nodes = n;
+    rl.cap_edges = m;
+  }
+  // bits of a vertex id
+  int B = 1;
+  while (B < 31 && ((size_t)1 << B) < n) B++;
+  DevBuf deg, ids, ndeg, k0, k1, v0, v1, tmp;
+  if (hipMalloc(&deg.p, n * 4) || hipMalloc(&ids.p, n * 4) || hipMalloc(&ndeg.p, (n + 1) * 4) ||
+      hipMalloc(&k0.p, m * 8) || hipMalloc(&k1.p, m * 8) || hipMalloc(&v0.p, m * 4) || hipMalloc(&v1.p, m * 4))
+    return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of the relabelling buffers failed");
+  const int nb = (int)((n + 255) / 256);
+  hipLaunchKernelGGL(k_rl_deg, dim3(nb), dim3(256), 0, s, nodes, d_nindex, (int*)deg.p, (int*)ids.p);
+  RLCHK(hipGetLastError());
+  // vertices by decreasing degree; the radix sort is stable, so equal
+  // degrees keep the caller's order
+  size_t b_deg = 0, b_scan = 0, b_key = 0;
+  hipcub::DoubleBuffer<u64> dk((u64*)k0.p, (u64*)k1.p);
+  hipcub::DoubleBuffer<int> dv((int*)v0.p, (int*)v1.p);
+  RLCHK(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, b_deg, (const int*)deg.p, (int*)ndeg.p,
+                                                     (const int*)ids.p, rl.order, nodes, 0, 32, s));
+  RLCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, b_scan, (const int*)ndeg.p, rl.nindex, nodes + 1, s));
+  RLCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b_key, dk, dv, edges, 0, 2 * B, s));
+  if (hipMalloc(&tmp.p, std::max({b_deg, b_scan, b_key, (size_t)16})))
+    return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of the sort storage failed");
+  RLCHK(hipcub::DeviceRadixSort::SortPairsDescending(tmp.p, b_deg, (const int*)deg.p, (int*)ndeg.p,
+                                                     (const int*)ids.p, rl.order, nodes, 0, 32, s));
+  hipLaunchKernelGGL(k_rl_newid, dim3(nb), dim3(256), 0, s, nodes, rl.order, rl.newid, (int*)ndeg.p);
+  RLCHK(hipGetLastError());
+  // offsets of the new lists (ndeg holds the degrees in the new order)
+  RLCHK(hipcub::DeviceScan::ExclusiveSum(tmp.p, b_scan, (const int*)ndeg.p, rl.nindex, nodes + 1, s));
+  const long long kb = ((long long)edges + kKeyPer - 1) / kKeyPer;
+  hipLaunchKernelGGL(k_rl_keys, dim3((unsigned)kb), dim3(256), 0, s, nodes, edges, B, d_nindex, d_nlist, rl.newid,
+                     (u64*)k0.p, (int*)v0.p);
+  RLCHK(hipGetLastError());
+  RLCHK(hipcub::DeviceRadixSort::SortPairs(tmp.p, b_key, dk, dv, edges, 0, 2 * B, s));
+  const int fg = (int)std::min<long long>(((long long)edges + 255) / 256, (long long)ctx->cus * 32);
+  hipLaunchKernelGGL(k_rl_fill, dim3(fg), dim3(256), 0, s, edges, B, (const u64*)dk.Current(),
+                     (const int*)dv.Current(), rl.order, d_nlist, d_eweight, rev, rl.nlist, rl.eweight, rl.kid);
+  RLCHK(hipGetLastError());
+  RLCHK(hipStreamSynchronize(s));
+#undef RLCHK
+  rl.ok = true;
+  rl.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return EMST_OK;
+}
+
+}  // namespace emst

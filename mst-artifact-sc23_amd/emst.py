@@ -29,6 +29,8 @@ F_NO_COMBINE = 4
 F_NO_PAIR_DEDUP = 1024
 F_NO_SPARSE_HEAVY = 2048
 F_NO_LIGHT_OFFERS = 4096
+F_NO_RELABEL = 8192
+F_RELABEL = 16384
 # the de-optimization ladder (Deoptimizations/ECL_MST_No*.cu)
 F_LADDER = 8
 F_NO_WARP = 16
@@ -91,7 +93,7 @@ class Stats(C.Structure):
                 ("light", C.c_int64), ("heavy", C.c_int64), ("mst_edges", C.c_int64),
                 ("weight", C.c_uint64), ("round_in_sum", C.c_int64), ("round_out_sum", C.c_int64),
                 ("threshold_used", C.c_int32), ("filter_used", C.c_int32), ("learned_q", C.c_float),
-                ("sparse_heavy", C.c_int32), ("light_offers", C.c_int32)]
+                ("sparse_heavy", C.c_int32), ("light_offers", C.c_int32), ("relabelled", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

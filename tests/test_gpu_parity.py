@@ -185,10 +185,11 @@ def test_modes_agree(solver):
 def test_repeat_and_roots(solver):
     g = emst.gen_rmat(14, 8, 0.57, 0.19, 0.19, 12, 1, 100)
     uf, T = emst.plain_threshold(g)
-    solver.load(g)
+    # in the caller's numbering (a skewed graph is renumbered by default)
+    with_flags(solver, emst.F_NO_RELABEL, lambda: solver.load(g))
     outs = []
     for _ in range(3):
-        solver.solve(uf, T)
+        assert solver.solve(uf, T).relabelled == 0
         outs.append(solver.fetch())
     assert all(np.array_equal(outs[0], o) for o in outs)
     parent = solver.fetch_parent(g.nodes)
@@ -213,6 +214,24 @@ def test_repeat_and_roots(solver):
             p[max(a, b)] = min(a, b)
     comp = np.array([f(x) for x in range(g.nodes)])
     assert np.array_equal(roots, comp)
+    # renumbered by degree: the same forest; the parent array comes back in
+    # the caller's ids, each tree's root is one of its members (the one of
+    # largest degree) and a fixed point
+    solver.load(g)
+    st = solver.solve(uf, T)
+    assert st.relabelled == 1
+    assert np.array_equal(solver.fetch(), outs[0])
+    parent = solver.fetch_parent(g.nodes)
+    r2 = parent.copy()
+    for _ in range(64):
+        r2 = r2[r2]
+    assert np.array_equal(r2[r2], r2)
+    assert np.array_equal(comp[r2], comp)       # each vertex's root lies in its own tree
+    assert len(np.unique(r2)) == len(np.unique(comp))  # one root per tree
+    deg = np.diff(g.nindex)
+    for c in np.unique(comp)[:50]:
+        members = np.flatnonzero(comp == c)
+        assert deg[r2[members[0]]] == deg[members].max()
 
 
 def test_solve_device_pointers(solver):
@@ -412,3 +431,65 @@ def test_sparse_heavy_falls_back_on_unsorted_or_one_way_lists(solver):
         for g in (g1, g2):
             for T in (3, 12):
                 check(solver, g, True, T, rounds=False, kruskal=g is g1)
+
+
+def test_relabelled_layout_matches_caller_numbering(solver):
+    """emst_load renumbers a graph with sorted, symmetric lists by decreasing
+    degree (by default when the degrees are skewed; EMST_F_RELABEL forces it)
+    and solves on the rebuilt CSR with every position keyed by its edge's
+    upper-direction position in the caller's CSR.  Bytes, L, H, rounds and,
+    with parallel records kept, every round's worklist sizes are the
+    oracle's and the same as in the caller's numbering (EMST_F_NO_RELABEL):
+    ties, parallel edges, self-loops, unmirrored weights, filtered,
+    unfiltered and learned."""
+    rng = np.random.default_rng(41)
+    cases = _sparse_cases()
+    for i in range(4):
+        ni, nl, ew = random_graph(rng, 300, 1500, 1, 1000)
+        src = np.repeat(np.arange(300), np.diff(ni))
+        ew = ew.copy()
+        low = nl < src
+        ew[low] = rng.integers(1, 1000, int(low.sum()))  # the copies disagree: the upper one counts
+        cases.append((f"unmirrored{i}", G(ni, nl, ew)))
+    ran = 0
+    for name, g in cases:
+        uf, T = emst.plain_threshold(g)
+        w = np.sort(g.eweight) if g.edges else np.zeros(1, np.int32)
+        for f, T2 in ((uf, T), (True, int(w[len(w) // 3])), (False, emst.INT_MAX)):
+            got, st = with_flags(solver, emst.F_RELABEL, lambda: check(solver, g, f, T2))
+            got2, st2 = with_flags(solver, emst.F_NO_RELABEL, lambda: solver.gpu_mst(g, f, T2))
+            assert np.array_equal(got, got2), name
+            assert (st.light, st.heavy, st.light_rounds, st.heavy_rounds) == \
+                (st2.light, st2.heavy, st2.light_rounds, st2.heavy_rounds), name
+            assert st2.relabelled == 0
+            ran += st.relabelled
+        # learned threshold on the renumbered copy: the same bytes as in the
+        # caller's numbering
+        with_flags(solver, emst.F_RELABEL, lambda: solver.load(g))
+        st = solver.solve(emst.FILTER_LEARNED, 0)
+        got = solver.fetch()
+        with_flags(solver, emst.F_NO_RELABEL, lambda: solver.load(g))
+        st2 = solver.solve(emst.FILTER_LEARNED, 0)
+        assert np.array_equal(got, solver.fetch()), name
+        assert (st.light, st.heavy, st.light_rounds, st.heavy_rounds, st.threshold_used) == \
+            (st2.light, st2.heavy, st2.light_rounds, st2.heavy_rounds, st2.threshold_used), name
+        assert np.array_equal(got, O.kruskal(g)), name
+    # every case has sorted, symmetric lists: each one with an edge was renumbered
+    assert ran == 3 * sum(1 for _, g in cases if g.edges > 0 and g.nodes >= 2)
+
+
+def test_relabel_default_only_for_skewed_degrees(solver):
+    """By default only a graph whose largest degree is at least 32 times the
+    mean is renumbered; lists without reverse positions never are."""
+    rmat = emst.gen_rmat(14, 16, 0.57, 0.19, 0.19, 2, 1, 1 << 24)
+    for g, want in ((rmat, 1), (emst.gen_grid(90, 70, 4, 1, 50), 0), (emst.gen_rgg(20000, 6.0, 3, 1 << 20), 0),
+                    (emst.gen_er(20000, 10.0, 4, 1, 1 << 20), 0)):
+        solver.load(g)
+        uf, T = emst.plain_threshold(g)
+        assert solver.solve(uf, T).relabelled == want
+    rng = np.random.default_rng(43)
+    edges = [(int(a), int(b), int(c)) for a, b, c in zip(rng.integers(0, 2000, 9000), rng.integers(0, 2000, 9000),
+                                                         rng.integers(1, 30, 9000))]
+    g = G(*csr_from_edges(2000, edges, symmetric=True, sort=False))
+    got, st = with_flags(solver, emst.F_RELABEL, lambda: solver.gpu_mst(g, True, 10))
+    assert st.relabelled == 0 and np.array_equal(got, O.kruskal(g))
