@@ -255,9 +255,12 @@ int emst_solve_device_noalloc(emst_ctx* ctx, int32_t nodes, int32_t edges, const
                               const int32_t* d_nlist, const int32_t* d_eweight, int use_filter,
                               int32_t threshold, uint8_t* d_inMST, void* stream, emst_stats* stats);
 
-/* Device parent array of the last solve (ints, nodes entries): after a
- * solve every vertex's root is the smallest vertex id of its tree
- * (join() at ECL_MST.cu:137-144 hooks the larger root under the smaller). */
+/* Parent array of the last solve (ints, nodes entries), copied to host
+ * memory in the caller's numbering: after a solve every vertex's root is
+ * the smallest vertex id of its tree in the numbering the solve ran in
+ * (join() at ECL_MST.cu:137-144 hooks the larger root under the smaller).
+ * For a graph solved on its degree-ordered copy (stats.relabelled) that is
+ * the tree's member of largest degree (ties: the smallest caller id). */
 int emst_fetch_parent(emst_ctx* ctx, int32_t* parent);
 
 /* ---- batched solving (no counterpart in the reference) --------------------
