@@ -188,7 +188,7 @@ def run_single(a, emst):
     # A graph with skewed degrees is solved on a copy renumbered by degree,
     # built by emst_load: the same solve in the caller's numbering, and the
     # load without the copy, for comparison
-    layout = {"relabelled": bool(st_l[-1].relabelled)}
+    layout = {"relabelled": ["none", "degree order", "cluster order"][st_l[-1].relabelled]}
     if st_l[-1].relabelled:
         saved = s.config().as_dict()
         s.set_config(**{**saved, "flags": saved["flags"] | emst.F_NO_RELABEL})
@@ -477,7 +477,7 @@ def run_sub_single(a, emst, name):
                 "filter": bool(st.filter_used), "light": st.light, "heavy": st.heavy,
                 "rounds": [st.light_rounds, st.heavy_rounds], "init_light_ms": st.init_light_ms,
                 "init_heavy_ms": st.init_heavy_ms, "sparse_heavy": bool(st.sparse_heavy),
-                "relabelled": bool(st.relabelled)}
+                "relabelled": ["none", "degree order", "cluster order"][st.relabelled]}
     return {"workload": w["desc"], "nodes": g.nodes, "edges": g.edges,
             "learned": {**rec(ms_l, st_l), "learned_q": st_l[-1].learned_q}, "plain": rec(ms_p, st_p),
             "learned_vs_plain": ratio, "learned_vs_plain_blocks": ms_p / ms_l,

@@ -88,7 +88,8 @@ typedef struct {
   float learned_q;           /* quantile predicted by the network, -1 if not used      */
   int32_t sparse_heavy;      /* 1 when the sparse heavy initialisation ran             */
   int32_t light_offers;      /* 1 when the light initialisation made the first offers  */
-  int32_t relabelled;        /* 1 when the solve ran on the degree-ordered copy         */
+  int32_t relabelled;        /* renumbered copy the solve ran on: 0 none, 1 by degree,   */
+                             /* 2 by breadth-first clusters                              */
 } emst_stats;
 
 /* Tunables.  Zero in a field selects the built-in default. */
@@ -130,17 +131,24 @@ typedef struct {
  * Results, records and round counts are the same either way (ablation). */
 #define EMST_F_NO_LIGHT_OFFERS 4096
 /* Keep the caller's vertex numbering.  By default emst_load renumbers a
- * graph with skewed degrees (largest degree at least 32 times the mean) and
- * sorted, symmetric lists by decreasing degree and solves on a copy of its
- * CSR in that numbering (rebuilt on the device, once per load): the hubs'
- * per-vertex entries then share cache lines.  Keys, records and inMST keep
- * the caller's edge positions, so results and round counts are the same
- * either way; only the root ids that emst_fetch_parent reports differ.
- * Read at emst_load (ablation). */
+ * graph with sorted, symmetric lists and solves on a copy of its CSR in the
+ * new numbering (rebuilt on the device, once per load):
+ *  - skewed degrees (largest degree at least 32 times the mean): by
+ *    decreasing degree, so the hubs' per-vertex entries share cache lines;
+ *  - scattered ids (under a tenth of the edges join ids at most 64 apart,
+ *    at least 65536 vertices): by clusters grown breadth-first from every
+ *    4096th vertex, ordered by (cluster, level), when the search takes at
+ *    least 16 levels (a graph of high diameter, such as a geometric one).
+ * Keys, records and inMST keep the caller's edge positions, so results and
+ * round counts are the same either way; only the root ids that
+ * emst_fetch_parent reports differ.  Read at emst_load (ablation). */
 #define EMST_F_NO_RELABEL 8192
-/* Renumber at emst_load whenever the lists are sorted and symmetric,
- * whatever the degrees (testing). */
+/* Renumber by decreasing degree at emst_load whenever the lists are sorted
+ * and symmetric, whatever the degrees (testing). */
 #define EMST_F_RELABEL 16384
+/* Renumber in cluster order at emst_load whenever the lists are sorted and
+ * symmetric, whatever the ids and the diameter (testing). */
+#define EMST_F_CLUSTER 32768
 
 /* The de-optimization ladder (reference: Deoptimizations/ECL_MST_No*.cu,
  * removing one optimization per file).  EMST_F_LADDER runs the reference's

@@ -18,7 +18,9 @@ struct Relabel {
   int* order = nullptr;
   int* newid = nullptr;
   size_t cap_nodes = 0, cap_edges = 0;
-  double ms = 0;  // build time at the last load
+  double ms = 0;   // build time at the last load
+  int mode = 0;    // the order in use: emst::kOrderDegree or emst::kOrderCluster
+  int levels = 0;  // breadth-first levels of the cluster search (cluster order)
 };
 
 struct emst_ctx {
@@ -189,10 +191,14 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
 int build_rev(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
               const int* d_eweight, int* rev, hipStream_t s, bool* ok, bool* mirrored);
 
-// Builds the context's degree-ordered layout (ctx->rl) of the loaded graph
-// from its device copy and reverse positions (relabel.hip); sets rl.ok.
+// Builds the context's renumbered layout (ctx->rl) of the loaded graph from
+// its device copy and reverse positions (relabel.hip); sets rl.ok.
+// kOrderDegree: by decreasing degree.  kOrderCluster: by breadth-first
+// clusters grown from every 4096th vertex, kept only when the search takes
+// at least min_levels levels (a graph of high diameter).
+constexpr int kOrderDegree = 1, kOrderCluster = 2;
 int build_relabel(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
-                  const int* d_eweight, const int* rev, hipStream_t s);
+                  const int* d_eweight, const int* rev, hipStream_t s, int mode, int min_levels);
 void free_relabel(Relabel& rl);
 // Copies a parent array of the degree-ordered layout to host memory in the
 // caller's numbering.
@@ -215,6 +221,7 @@ constexpr int kLadderFlags = EMST_F_LADDER | EMST_F_NO_WARP | EMST_F_NO_IMPLICIT
                              EMST_F_NO_TUPLES | EMST_F_NO_DATA_DRIVEN | EMST_F_NO_EDGE_BASED;
 constexpr int kAllFlags = EMST_F_HOST_SYNC | EMST_F_NO_GUARD | EMST_F_NO_COMBINE | EMST_F_NO_PAIR_DEDUP |
                           EMST_F_NO_SPARSE_HEAVY | EMST_F_NO_LIGHT_OFFERS | EMST_F_NO_RELABEL | EMST_F_RELABEL |
+                          EMST_F_CLUSTER |
                           kLadderFlags;
 
 }  // namespace emst

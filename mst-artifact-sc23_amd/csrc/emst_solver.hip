@@ -1881,7 +1881,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
     stats->round_out_sum = hs[ST_OUT_SUM];
     stats->sparse_heavy = (int32_t)hs[ST_COUNT + 3];
     stats->light_offers = (int32_t)hs[ST_COUNT + 4];
-    stats->relabelled = kidmode;
+    stats->relabelled = kidmode ? ctx->rl.mode : 0;
   }
   return EMST_OK;
 }
@@ -2041,25 +2041,29 @@ int emst_load(emst_ctx* ctx, const emst_graph* g)
   // Lower-direction positions are counted too: reverse positions need as
   // many of them as upper-direction ones.
   const int nt = emst::host_threads();
-  std::vector<long long> part((size_t)nt, 0), part_lo((size_t)nt, 0), part_max((size_t)nt, 0);
+  std::vector<long long> part((size_t)nt, 0), part_lo((size_t)nt, 0), part_max((size_t)nt, 0),
+      part_near((size_t)nt, 0);
   emst::parallel_for(g->nodes, [&](int64_t b, int64_t e, int t) {
-    long long c = 0, lo = 0, mx = 0;
+    long long c = 0, lo = 0, mx = 0, near = 0;
     for (int64_t v = b; v < e; v++) {
       for (int32_t j = g->nindex[v]; j < g->nindex[v + 1]; j++) {
         c += g->nlist[j] > v;
         lo += g->nlist[j] < v;
+        near += g->nlist[j] != v && std::llabs((long long)g->nlist[j] - v) <= 64;
       }
       mx = std::max<long long>(mx, g->nindex[v + 1] - g->nindex[v]);
     }
     part[t] = c;
     part_lo[t] = lo;
     part_max[t] = mx;
+    part_near[t] = near;
   }, nt);
-  long long up = 0, low = 0, maxdeg = 0;
+  long long up = 0, low = 0, maxdeg = 0, near = 0;
   for (int t = 0; t < nt; t++) {
     up += part[t];
     low += part_lo[t];
     maxdeg = std::max(maxdeg, part_max[t]);
+    near += part_near[t];
   }
   ctx->loaded = false;
   ctx->learned_cached = false;
@@ -2082,16 +2086,32 @@ int emst_load(emst_ctx* ctx, const emst_graph* g)
                         &ctx->rev_ok, &ctx->rev_mirrored);
     if (rc2) return rc2;
   }
-  // Skewed degrees (the largest at least 32 times the mean): solve on a copy
-  // renumbered by decreasing degree.  It needs the reverse positions, which
-  // give every lower-direction copy its key id.
+  // Solve on a renumbered copy (it needs the reverse positions, which give
+  // every lower-direction copy its key id):
+  //  - skewed degrees (the largest at least 32 times the mean): by
+  //    decreasing degree, so the hubs' entries share lines;
+  //  - scattered ids (under a tenth of the edges join ids at most 64 apart)
+  //    on a graph of high diameter: by breadth-first clusters, so most
+  //    edges join nearby ids.
   ctx->rl.ok = false;
   ctx->rl.ms = 0;
+  ctx->rl.mode = 0;
   const int fl = ctx->cfg.flags;
   const bool skewed = g->nodes >= 1024 && maxdeg * (long long)g->nodes >= 32LL * g->edges;
-  if (ctx->rev_ok && !(fl & EMST_F_NO_RELABEL) && ((fl & EMST_F_RELABEL) || skewed)) {
+  const bool scattered = g->nodes >= 65536 && near * 10 < (long long)g->edges;
+  int order = 0, min_levels = 0;
+  if (ctx->rev_ok && !(fl & EMST_F_NO_RELABEL)) {
+    if (fl & EMST_F_RELABEL) order = kOrderDegree;
+    else if (fl & EMST_F_CLUSTER) order = kOrderCluster;
+    else if (skewed) order = kOrderDegree;
+    else if (scattered) {
+      order = kOrderCluster;
+      min_levels = 16;
+    }
+  }
+  if (order) {
     const int rc3 = build_relabel(ctx, g->nodes, g->edges, ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, ctx->d_rev,
-                                  ctx->stream);
+                                  ctx->stream, order, min_levels);
     if (rc3) return rc3;
   }
   HIPCHK(hipStreamSynchronize(ctx->stream));

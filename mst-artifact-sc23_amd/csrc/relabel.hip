@@ -35,6 +35,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cstdio>
 
 #include "emst_internal.hpp"
@@ -132,6 +133,74 @@ __global__ __launch_bounds__(256) void k_rl_parent(int n, const int* __restrict_
   if (v < n) out[v] = order[parent[newid[v]]];
 }
 
+// Cluster order: a breadth-first search from every kClusterStride-th vertex
+// at once.  A vertex takes the smallest seed label among its neighbours
+// reached one level earlier (push, then take: deterministic whatever the
+// schedule); vertices are then ordered by (label, level, caller id).
+constexpr int kClusterStride = 4096;
+constexpr int kUnreached = INT_MAX;
+constexpr int kMaxLevels = 1 << 16;
+
+__global__ __launch_bounds__(256) void k_cl_init(int n, int* __restrict__ lab, int* __restrict__ lev,
+                                                 int* __restrict__ cand)
+{
+  const int v = blockIdx.x * 256 + threadIdx.x;
+  if (v < n) {
+    const bool seed = v % kClusterStride == 0;
+    lab[v] = seed ? v / kClusterStride : kUnreached;
+    lev[v] = seed ? 0 : kUnreached;
+    cand[v] = kUnreached;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cl_push(int n, int t, const int* __restrict__ nindex,
+                                                 const int* __restrict__ nlist, const int* __restrict__ lab,
+                                                 const int* __restrict__ lev, int* __restrict__ cand)
+{
+  const int u = blockIdx.x * 256 + threadIdx.x;
+  if (u >= n || lev[u] != t - 1) return;
+  const int l = lab[u];
+  for (int p = nindex[u]; p < nindex[u + 1]; p++) {
+    const int v = nlist[p];
+    if (lev[v] == kUnreached) atomicMin(&cand[v], l);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cl_take(int n, int t, int* __restrict__ lab, int* __restrict__ lev,
+                                                 const int* __restrict__ cand, int* __restrict__ count)
+{
+  const int v = blockIdx.x * 256 + threadIdx.x;
+  bool took = false;
+  if (v < n && lev[v] == kUnreached && cand[v] != kUnreached) {
+    lab[v] = cand[v];
+    lev[v] = t;
+    took = true;
+  }
+  const u64 b = __ballot(took);
+  if (emst::lane_id() == 0 && b) atomicAdd(count, __popcll(b));
+}
+
+// Sort keys of the cluster order: (label, level); unreached vertices last.
+__global__ __launch_bounds__(256) void k_cl_keys(int n, int nseeds, const int* __restrict__ lab,
+                                                 const int* __restrict__ lev, u64* __restrict__ key,
+                                                 int* __restrict__ ids)
+{
+  const int v = blockIdx.x * 256 + threadIdx.x;
+  if (v < n) {
+    const bool r = lev[v] != kUnreached;
+    key[v] = ((u64)(r ? lab[v] : nseeds) << 32) | (u64)(unsigned)(r ? lev[v] : 0);
+    ids[v] = v;
+  }
+}
+
+// Degrees in the new order.
+__global__ __launch_bounds__(256) void k_rl_ndeg(int n, const int* __restrict__ order, const int* __restrict__ nindex,
+                                                 int* __restrict__ ndeg)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) ndeg[i] = nindex[order[i] + 1] - nindex[order[i]];
+}
+
 struct DevBuf {
   void* p = nullptr;
   ~DevBuf()
@@ -167,10 +236,12 @@ int relabel_parent(emst_ctx* ctx, const int* parent, int32_t nodes, int32_t* out
 }
 
 int build_relabel(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
-                  const int* d_eweight, const int* rev, hipStream_t s)
+                  const int* d_eweight, const int* rev, hipStream_t s, int mode, int min_levels)
 {
   Relabel& rl = ctx->rl;
   rl.ok = false;
+  rl.mode = 0;
+  rl.levels = 0;
   if (nodes < 2 || edges < 1) return EMST_OK;
   const auto t0 = std::chrono::steady_clock::now();
   auto fail = [&](hipError_t e, const char* what) {
@@ -198,26 +269,65 @@ int build_relabel(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nind
   // bits of a vertex id
   int B = 1;
   while (B < 31 && ((size_t)1 << B) < n) B++;
-  DevBuf deg, ids, ndeg, k0, k1, v0, v1, tmp;
+  const int nseeds = (int)((n + kClusterStride - 1) / kClusterStride);
+  int SB = 1;  // bits of a cluster label (unreached vertices take nseeds)
+  while (SB < 31 && ((size_t)1 << SB) <= (size_t)nseeds) SB++;
+  DevBuf deg, ids, ndeg, k0, k1, v0, v1, tmp, lab, lev, cand, vk0, vk1, cnt;
   if (hipMalloc(&deg.p, n * 4) || hipMalloc(&ids.p, n * 4) || hipMalloc(&ndeg.p, (n + 1) * 4) ||
       hipMalloc(&k0.p, m * 8) || hipMalloc(&k1.p, m * 8) || hipMalloc(&v0.p, m * 4) || hipMalloc(&v1.p, m * 4))
     return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of the relabelling buffers failed");
+  if (mode == kOrderCluster && (hipMalloc(&lab.p, n * 4) || hipMalloc(&lev.p, n * 4) || hipMalloc(&cand.p, n * 4) ||
+                                hipMalloc(&vk0.p, n * 8) || hipMalloc(&vk1.p, n * 8) || hipMalloc(&cnt.p, 4)))
+    return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of the cluster search buffers failed");
   const int nb = (int)((n + 255) / 256);
   hipLaunchKernelGGL(k_rl_deg, dim3(nb), dim3(256), 0, s, nodes, d_nindex, (int*)deg.p, (int*)ids.p);
   RLCHK(hipGetLastError());
-  // vertices by decreasing degree; the radix sort is stable, so equal
-  // degrees keep the caller's order
-  size_t b_deg = 0, b_scan = 0, b_key = 0;
+  size_t b_deg = 0, b_scan = 0, b_key = 0, b_cl = 0;
   hipcub::DoubleBuffer<u64> dk((u64*)k0.p, (u64*)k1.p);
   hipcub::DoubleBuffer<int> dv((int*)v0.p, (int*)v1.p);
   RLCHK(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, b_deg, (const int*)deg.p, (int*)ndeg.p,
                                                      (const int*)ids.p, rl.order, nodes, 0, 32, s));
   RLCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, b_scan, (const int*)ndeg.p, rl.nindex, nodes + 1, s));
   RLCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b_key, dk, dv, edges, 0, 2 * B, s));
-  if (hipMalloc(&tmp.p, std::max({b_deg, b_scan, b_key, (size_t)16})))
+  if (mode == kOrderCluster)
+    RLCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b_cl, (const u64*)vk0.p, (u64*)vk1.p, (const int*)ids.p,
+                                             rl.order, nodes, 0, 32 + SB, s));
+  if (hipMalloc(&tmp.p, std::max({b_deg, b_scan, b_key, b_cl, (size_t)16})))
     return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of the sort storage failed");
-  RLCHK(hipcub::DeviceRadixSort::SortPairsDescending(tmp.p, b_deg, (const int*)deg.p, (int*)ndeg.p,
-                                                     (const int*)ids.p, rl.order, nodes, 0, 32, s));
+  if (mode == kOrderCluster) {
+    // breadth-first levels from all seeds at once; one host read per level
+    hipLaunchKernelGGL(k_cl_init, dim3(nb), dim3(256), 0, s, nodes, (int*)lab.p, (int*)lev.p, (int*)cand.p);
+    RLCHK(hipGetLastError());
+    int t = 1;
+    for (;; t++) {
+      int got = 0;
+      RLCHK(hipMemsetAsync(cnt.p, 0, 4, s));
+      hipLaunchKernelGGL(k_cl_push, dim3(nb), dim3(256), 0, s, nodes, t, d_nindex, d_nlist, (const int*)lab.p,
+                         (const int*)lev.p, (int*)cand.p);
+      hipLaunchKernelGGL(k_cl_take, dim3(nb), dim3(256), 0, s, nodes, t, (int*)lab.p, (int*)lev.p,
+                         (const int*)cand.p, (int*)cnt.p);
+      RLCHK(hipGetLastError());
+      RLCHK(hipMemcpyAsync(&got, cnt.p, 4, hipMemcpyDeviceToHost, s));
+      RLCHK(hipStreamSynchronize(s));
+      if (got == 0 || t == kMaxLevels) break;  // (vertices still unreached go last)
+    }
+    rl.levels = t - 1;
+    // a low-diameter graph gains nothing from clusters: keep the caller's numbering
+    if (rl.levels < min_levels) return EMST_OK;
+    hipLaunchKernelGGL(k_cl_keys, dim3(nb), dim3(256), 0, s, nodes, nseeds, (const int*)lab.p, (const int*)lev.p,
+                       (u64*)vk0.p, (int*)ids.p);
+    RLCHK(hipGetLastError());
+    // stable: within a level of a cluster, the caller's order
+    RLCHK(hipcub::DeviceRadixSort::SortPairs(tmp.p, b_cl, (const u64*)vk0.p, (u64*)vk1.p, (const int*)ids.p,
+                                             rl.order, nodes, 0, 32 + SB, s));
+    hipLaunchKernelGGL(k_rl_ndeg, dim3(nb), dim3(256), 0, s, nodes, rl.order, d_nindex, (int*)ndeg.p);
+    RLCHK(hipGetLastError());
+  } else {
+    // vertices by decreasing degree; the radix sort is stable, so equal
+    // degrees keep the caller's order
+    RLCHK(hipcub::DeviceRadixSort::SortPairsDescending(tmp.p, b_deg, (const int*)deg.p, (int*)ndeg.p,
+                                                       (const int*)ids.p, rl.order, nodes, 0, 32, s));
+  }
   hipLaunchKernelGGL(k_rl_newid, dim3(nb), dim3(256), 0, s, nodes, rl.order, rl.newid, (int*)ndeg.p);
   RLCHK(hipGetLastError());
   // offsets of the new lists (ndeg holds the degrees in the new order)
@@ -234,6 +344,7 @@ int build_relabel(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nind
   RLCHK(hipStreamSynchronize(s));
 #undef RLCHK
   rl.ok = true;
+  rl.mode = mode;
   rl.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return EMST_OK;
 }

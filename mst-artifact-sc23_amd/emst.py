@@ -31,6 +31,7 @@ F_NO_SPARSE_HEAVY = 2048
 F_NO_LIGHT_OFFERS = 4096
 F_NO_RELABEL = 8192
 F_RELABEL = 16384
+F_CLUSTER = 32768
 # the de-optimization ladder (Deoptimizations/ECL_MST_No*.cu)
 F_LADDER = 8
 F_NO_WARP = 16

@@ -433,11 +433,14 @@ def test_sparse_heavy_falls_back_on_unsorted_or_one_way_lists(solver):
                 check(solver, g, True, T, rounds=False, kruskal=g is g1)
 
 
-def test_relabelled_layout_matches_caller_numbering(solver):
-    """emst_load renumbers a graph with sorted, symmetric lists by decreasing
-    degree (by default when the degrees are skewed; EMST_F_RELABEL forces it)
-    and solves on the rebuilt CSR with every position keyed by its edge's
-    upper-direction position in the caller's CSR.  Bytes, L, H, rounds and,
+@pytest.mark.parametrize("flag,mode", [(emst.F_RELABEL, 1), (emst.F_CLUSTER, 2)], ids=["degree", "cluster"])
+def test_relabelled_layout_matches_caller_numbering(solver, flag, mode):
+    """emst_load renumbers a graph with sorted, symmetric lists (by
+    decreasing degree when the degrees are skewed, in breadth-first cluster
+    order when the ids are scattered and the diameter is high;
+    EMST_F_RELABEL and EMST_F_CLUSTER force either) and solves on the rebuilt
+    CSR with every position keyed by its edge's upper-direction position in
+    the caller's CSR.  Bytes, L, H, rounds and,
     with parallel records kept, every round's worklist sizes are the
     oracle's and the same as in the caller's numbering (EMST_F_NO_RELABEL):
     ties, parallel edges, self-loops, unmirrored weights, filtered,
@@ -456,16 +459,17 @@ def test_relabelled_layout_matches_caller_numbering(solver):
         uf, T = emst.plain_threshold(g)
         w = np.sort(g.eweight) if g.edges else np.zeros(1, np.int32)
         for f, T2 in ((uf, T), (True, int(w[len(w) // 3])), (False, emst.INT_MAX)):
-            got, st = with_flags(solver, emst.F_RELABEL, lambda: check(solver, g, f, T2))
+            got, st = with_flags(solver, flag, lambda: check(solver, g, f, T2))
             got2, st2 = with_flags(solver, emst.F_NO_RELABEL, lambda: solver.gpu_mst(g, f, T2))
             assert np.array_equal(got, got2), name
             assert (st.light, st.heavy, st.light_rounds, st.heavy_rounds) == \
                 (st2.light, st2.heavy, st2.light_rounds, st2.heavy_rounds), name
             assert st2.relabelled == 0
-            ran += st.relabelled
+            assert st.relabelled in (0, mode), name
+            ran += st.relabelled == mode
         # learned threshold on the renumbered copy: the same bytes as in the
         # caller's numbering
-        with_flags(solver, emst.F_RELABEL, lambda: solver.load(g))
+        with_flags(solver, flag, lambda: solver.load(g))
         st = solver.solve(emst.FILTER_LEARNED, 0)
         got = solver.fetch()
         with_flags(solver, emst.F_NO_RELABEL, lambda: solver.load(g))
@@ -478,15 +482,23 @@ def test_relabelled_layout_matches_caller_numbering(solver):
     assert ran == 3 * sum(1 for _, g in cases if g.edges > 0 and g.nodes >= 2)
 
 
-def test_relabel_default_only_for_skewed_degrees(solver):
-    """By default only a graph whose largest degree is at least 32 times the
-    mean is renumbered; lists without reverse positions never are."""
+def test_relabel_default_choice(solver):
+    """By default a graph whose largest degree is at least 32 times the mean
+    is renumbered by degree; one of at least 65536 vertices whose edges
+    mostly join distant ids is renumbered in cluster order when its
+    breadth-first search is deep (a geometric graph), not when it is shallow
+    (Erdos-Renyi); local ids (a grid) and small graphs keep the caller's
+    numbering; lists without reverse positions never are renumbered."""
     rmat = emst.gen_rmat(14, 16, 0.57, 0.19, 0.19, 2, 1, 1 << 24)
     for g, want in ((rmat, 1), (emst.gen_grid(90, 70, 4, 1, 50), 0), (emst.gen_rgg(20000, 6.0, 3, 1 << 20), 0),
-                    (emst.gen_er(20000, 10.0, 4, 1, 1 << 20), 0)):
+                    (emst.gen_er(20000, 10.0, 4, 1, 1 << 20), 0), (emst.gen_grid(400, 400, 4, 1, 1 << 20), 0),
+                    (emst.gen_rgg(1 << 17, 6.0, 5, 1 << 20), 2), (emst.gen_er(1 << 17, 10.0, 6, 1, 1 << 20), 0)):
         solver.load(g)
         uf, T = emst.plain_threshold(g)
-        assert solver.solve(uf, T).relabelled == want
+        st = solver.solve(uf, T)
+        assert st.relabelled == want, (g.nodes, g.edges)
+        if want == 2:
+            assert np.array_equal(solver.fetch(), O.kruskal(g))
     rng = np.random.default_rng(43)
     edges = [(int(a), int(b), int(c)) for a, b, c in zip(rng.integers(0, 2000, 9000), rng.integers(0, 2000, 9000),
                                                          rng.integers(1, 30, 9000))]

@@ -48,7 +48,7 @@ def test_library_exports_every_declared_symbol():
 def test_flag_constants_match_header():
     text = open(os.path.join(ROOT, "include", "emst.h")).read()
     flags = {k: int(v) for k, v in re.findall(r"#define EMST_(F_\w+)\s+(\d+)", text)}
-    assert len(flags) == 15
+    assert len(flags) == 16
     for k, v in flags.items():
         assert getattr(emst, k) == v, k
     assert len(set(flags.values())) == len(flags) and all(v & (v - 1) == 0 for v in flags.values())
