@@ -288,9 +288,6 @@ typedef struct {
   int32_t heavy_rounds;
   int32_t replays;           /* arenas served by a hipGraph replay              */
   int32_t filtered_graphs;   /* graphs that used the light/heavy split          */
-  int32_t renumbered_graphs; /* graphs whose arena was renumbered (by degree or  */
-                             /* by clusters, as for emst_load; EMST_F_NO_RELABEL  */
-                             /* at emst_batch_create keeps every arena as given) */
 } emst_batch_stats;
 /* Validates and uploads the graphs (host arrays stay owned by the caller).
  * max_arena_edges caps the edges of one arena (0: 2^31-2, the int32

@@ -46,11 +46,6 @@ struct Chunk {
   long long* d_eoff = nullptr;     // per-graph edge offsets
   int* d_rev = nullptr;            // reverse positions (sorted, symmetric lists), else null
   bool rev_mirrored = false;
-  // renumbered arena (every graph in its vertex range, ordered by degree or
-  // by clusters as it suits): the CSR arrays above are then the renumbered
-  // ones, and kid holds each position's key id in the caller's arena layout
-  int* d_kid = nullptr;
-  int relabelled_graphs = 0;
   std::vector<long long> voff, eoff;
   // captured schedule
   hipGraphExec_t exec = nullptr;
@@ -79,19 +74,10 @@ struct emst_batch {
 
 namespace {
 
-struct DevArr {
-  void* p = nullptr;
-  ~DevArr()
-  {
-    if (p) (void)hipFree(p);
-  }
-};
-
 void free_chunk(Chunk& c)
 {
   for (void* p : {(void*)c.d_nindex, (void*)c.d_nlist, (void*)c.d_eweight, (void*)c.d_inMST, (void*)c.d_vgid,
-                  (void*)c.d_gthr, (void*)c.d_gflag, (void*)c.d_voff, (void*)c.d_eoff, (void*)c.d_rev,
-                  (void*)c.d_kid})
+                  (void*)c.d_gthr, (void*)c.d_gflag, (void*)c.d_voff, (void*)c.d_eoff, (void*)c.d_rev})
     if (p) (void)hipFree(p);
   if (c.exec) (void)hipGraphExecDestroy(c.exec);
   c = Chunk();
@@ -170,28 +156,20 @@ int emst_batch_create(emst_ctx* ctx, int32_t ngraphs, const emst_graph* graphs, 
     std::vector<int32_t> nindex((size_t)c.nodes + 1), nlist((size_t)std::max(c.edges, 1)),
         ew((size_t)std::max(c.edges, 1)), vgid((size_t)c.nodes);
     std::vector<long long> ups(G, 0), lows(G, 0);
-    std::vector<int32_t> gtype(G, 0);
     emst::parallel_for(G, [&](int64_t lo, int64_t hi, int) {
       for (int64_t k = lo; k < hi; k++) {
         const emst_graph& gg = graphs[c.g0 + k];
         const long long vo = c.voff[k], eo = c.eoff[k];
-        long long up = 0, low = 0, mx = 0, near = 0;
+        long long up = 0, low = 0;
         for (int32_t v = 0; v < gg.nodes; v++) {
           nindex[vo + v] = (int32_t)(eo + gg.nindex[v]);
           vgid[vo + v] = (int32_t)k;
           for (int32_t j = gg.nindex[v]; j < gg.nindex[v + 1]; j++) {
             up += gg.nlist[j] > v;
             low += gg.nlist[j] < v;
-            near += gg.nlist[j] != v && std::llabs((long long)gg.nlist[j] - v) <= 64;
           }
-          mx = std::max<long long>(mx, gg.nindex[v + 1] - gg.nindex[v]);
         }
         lows[k] = low;
-        // the order that suits the graph, by the rules of a single load
-        // (emst_load): decreasing degree for skewed degrees, clusters for
-        // scattered ids (kept if the search is deep)
-        if (gg.nodes >= 1024 && mx * (long long)gg.nodes >= 32LL * gg.edges) gtype[k] = kOrderDegree;
-        else if (gg.nodes >= 65536 && near * 10 < (long long)gg.edges) gtype[k] = kOrderCluster;
         for (int32_t j = 0; j < gg.edges; j++) {
           nlist[eo + j] = (int32_t)(vo + gg.nlist[j]);
           ew[eo + j] = gg.eweight[j];
@@ -241,53 +219,6 @@ int emst_batch_create(emst_ctx* ctx, int32_t ngraphs, const emst_graph* graphs, 
       bool rok = false, mir = false;
       rc = build_rev(ctx, c.nodes, c.edges, c.d_nindex, c.d_nlist, c.d_eweight, c.d_rev, cs, &rok, &mir);
       if (rc) break;
-      int want = 0;
-      bool any_cluster = false;
-      for (int32_t k = 0; k < G; k++) {
-        want += gtype[k] != 0;
-        any_cluster = any_cluster || gtype[k] == kOrderCluster;
-      }
-      if (rok && want && !(ctx->cfg.flags & EMST_F_NO_RELABEL)) {
-        // renumber the arena; the renumbered CSR replaces the uploaded one
-        // (each graph keeps its vertex and edge ranges, so the offsets and
-        // the vertex-graph map stay valid) and the key ids the reverse
-        // positions
-        DevArr d_gtype;
-        if (hipMalloc(&d_gtype.p, (size_t)G * 4) != hipSuccess ||
-            hipMemcpyAsync(d_gtype.p, gtype.data(), (size_t)G * 4, hipMemcpyHostToDevice, cs) != hipSuccess) {
-          rc = ctx_fail(ctx, EMST_E_HIP, "uploading the arena's orders failed");
-          break;
-        }
-        OrderSpec spec;
-        spec.mode = kOrderUnion;
-        spec.min_levels = 16;
-        spec.vgid = c.d_vgid;
-        spec.voff = c.d_voff;
-        spec.gtype = (const int*)d_gtype.p;
-        spec.ngraphs = G;
-        spec.any_cluster = any_cluster;
-        Relabel rl;
-        rc = build_relabel(ctx, rl, c.nodes, c.edges, c.d_nindex, c.d_nlist, c.d_eweight, c.d_rev, cs, spec);
-        if (rc) {
-          free_relabel(rl);
-          break;
-        }
-        for (void* q : {(void*)c.d_nindex, (void*)c.d_nlist, (void*)c.d_eweight, (void*)c.d_rev, (void*)rl.order,
-                        (void*)rl.newid})
-          (void)hipFree(q);
-        c.d_nindex = rl.nindex;
-        c.d_nlist = rl.nlist;
-        c.d_eweight = rl.eweight;
-        c.d_kid = rl.kid;
-        c.d_rev = nullptr;
-        c.rev_mirrored = true;
-        c.relabelled_graphs = 0;
-        for (int32_t k = 0; k < G; k++)
-          c.relabelled_graphs += gtype[k] == kOrderDegree ||
-                                 (gtype[k] == kOrderCluster && k < (int32_t)rl.gdepth.size() &&
-                                  rl.gdepth[k] >= spec.min_levels);
-        continue;
-      }
       if (!rok || !mir) {
         (void)hipFree(c.d_rev);
         c.d_rev = nullptr;
@@ -456,11 +387,7 @@ int emst_batch_solve(emst_batch* b, int use_filter, const int32_t* thresholds, i
       c.exec = nullptr;
     }
     RevIndex rix;
-    if (c.d_kid) {
-      rix.rev = c.d_kid;
-      rix.mirrored = true;  // both copies carry the upper copy's weight
-      rix.kid = true;
-    } else if (c.d_rev) {
+    if (c.d_rev) {
       rix.rev = c.d_rev;
       rix.mirrored = c.rev_mirrored;
     }
@@ -507,7 +434,6 @@ int emst_batch_solve(emst_batch* b, int use_filter, const int32_t* thresholds, i
     st->ms = total_ms;
     st->chunks = (int32_t)b->chunks.size();
     st->graphs = b->ngraphs;
-    for (const Chunk& c : b->chunks) st->renumbered_graphs += c.relabelled_graphs;
     st->nodes = b->total_nodes;
     st->edges = b->total_edges;
     int32_t nf = 0;

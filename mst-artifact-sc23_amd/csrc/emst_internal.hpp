@@ -5,7 +5,6 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
-#include <vector>
 
 #include "../../include/emst.h"
 #include "emst_kernels.hpp"
@@ -22,7 +21,6 @@ struct Relabel {
   double ms = 0;   // build time at the last load
   int mode = 0;    // the order in use: emst::kOrderDegree or emst::kOrderCluster
   int levels = 0;  // breadth-first levels of the cluster search (cluster order)
-  std::vector<int> gdepth;  // union order: the search's depth in each graph
 };
 
 struct emst_ctx {
@@ -193,29 +191,14 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
 int build_rev(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
               const int* d_eweight, int* rev, hipStream_t s, bool* ok, bool* mirrored);
 
-// How build_relabel orders the vertices.
-//  kOrderDegree: by decreasing degree.
-//  kOrderCluster: by breadth-first clusters grown from every 4096th vertex,
-//    ordered by (cluster, level); kept only when the search takes at least
-//    min_levels levels (a graph of high diameter).
-//  kOrderUnion: a union of disjoint graphs (vgid, voff as in a batch arena)
-//    keeps every graph in its vertex range and orders each as gtype says
-//    (0 keep, kOrderDegree, kOrderCluster; a cluster order whose search in
-//    that graph stays shallower than min_levels keeps the ids).
-constexpr int kOrderDegree = 1, kOrderCluster = 2, kOrderUnion = 3;
-struct OrderSpec {
-  int mode = kOrderDegree;
-  int min_levels = 0;
-  const int* vgid = nullptr;
-  const long long* voff = nullptr;
-  const int* gtype = nullptr;
-  int ngraphs = 0;
-  bool any_cluster = false;
-};
-// Builds a renumbered layout of a device-resident graph (relabel.hip) from
-// its CSR and reverse positions into rl; sets rl.ok when the order is kept.
-int build_relabel(emst_ctx* ctx, Relabel& rl, int32_t nodes, int32_t edges, const int* d_nindex,
-                  const int* d_nlist, const int* d_eweight, const int* rev, hipStream_t s, const OrderSpec& spec);
+// Builds the context's renumbered layout (ctx->rl) of the loaded graph from
+// its device copy and reverse positions (relabel.hip); sets rl.ok.
+// kOrderDegree: by decreasing degree.  kOrderCluster: by breadth-first
+// clusters grown from every 4096th vertex, kept only when the search takes
+// at least min_levels levels (a graph of high diameter).
+constexpr int kOrderDegree = 1, kOrderCluster = 2;
+int build_relabel(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
+                  const int* d_eweight, const int* rev, hipStream_t s, int mode, int min_levels);
 void free_relabel(Relabel& rl);
 // Copies a parent array of the degree-ordered layout to host memory in the
 // caller's numbering.

@@ -1659,7 +1659,7 @@ int run_solve(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, 
                             sparse_lds_bytes(std::max(64, cfg.stage_recs)) <= ctx->lds_per_block;
   // a relabelled layout keys every position through rev (its key ids)
   const int kidmode = rix && rix->rev && rix->kid ? 1 : 0;
-  if (kidmode && !rix->mirrored) return ctx_fail(ctx, EMST_E_ARG, "key ids need mirrored weights");
+  if (kidmode && (vt || !rix->mirrored)) return ctx_fail(ctx, EMST_E_ARG, "key ids need one graph with mirrored weights");
   const WorkSizes z = work_sizes(ctx->cus, cfg, nodes, edges, wl_cap);
   const bool learned = use_filter == EMST_FILTER_LEARNED && !vt;
   // a learned threshold decided before the solve (cached per loaded graph)
@@ -2110,11 +2110,8 @@ int emst_load(emst_ctx* ctx, const emst_graph* g)
     }
   }
   if (order) {
-    OrderSpec spec;
-    spec.mode = order;
-    spec.min_levels = min_levels;
-    const int rc3 = build_relabel(ctx, ctx->rl, g->nodes, g->edges, ctx->d_nindex, ctx->d_nlist, ctx->d_eweight,
-                                  ctx->d_rev, ctx->stream, spec);
+    const int rc3 = build_relabel(ctx, g->nodes, g->edges, ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, ctx->d_rev,
+                                  ctx->stream, order, min_levels);
     if (rc3) return rc3;
   }
   HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -134,64 +134,23 @@ __global__ __launch_bounds__(256) void k_rl_parent(int n, const int* __restrict_
 }
 
 // Cluster order: a breadth-first search from every kClusterStride-th vertex
-// (of each graph of a union that takes this order) at once.  A seed's label
-// is its own id; a vertex takes the smallest label among its neighbours
+// at once.  A vertex takes the smallest seed label among its neighbours
 // reached one level earlier (push, then take: deterministic whatever the
 // schedule); vertices are then ordered by (label, level, caller id).
 constexpr int kClusterStride = 4096;
 constexpr int kUnreached = INT_MAX;
 constexpr int kMaxLevels = 1 << 16;
 
-__global__ __launch_bounds__(256) void k_cl_init(int n, const int* __restrict__ vgid, const long long* __restrict__ voff,
-                                                 const int* __restrict__ gtype, int* __restrict__ lab,
-                                                 int* __restrict__ lev, int* __restrict__ cand)
+__global__ __launch_bounds__(256) void k_cl_init(int n, int* __restrict__ lab, int* __restrict__ lev,
+                                                 int* __restrict__ cand)
 {
   const int v = blockIdx.x * 256 + threadIdx.x;
   if (v < n) {
-    bool seed = v % kClusterStride == 0;
-    if (vgid) {
-      const int g = vgid[v];
-      seed = gtype[g] == emst::kOrderCluster && (v - voff[g]) % kClusterStride == 0;
-    }
-    lab[v] = seed ? v : kUnreached;
+    const bool seed = v % kClusterStride == 0;
+    lab[v] = seed ? v / kClusterStride : kUnreached;
     lev[v] = seed ? 0 : kUnreached;
     cand[v] = kUnreached;
   }
-}
-
-// Deepest level reached in each graph of a union.
-__global__ __launch_bounds__(256) void k_cl_depth(int n, const int* __restrict__ vgid, const int* __restrict__ lev,
-                                                  int* __restrict__ gdepth)
-{
-  const int v = blockIdx.x * 256 + threadIdx.x;
-  if (v < n && lev[v] != kUnreached && lev[v] > 0) atomicMax(&gdepth[vgid[v]], lev[v]);
-}
-
-// Sort keys of a union's order: graph first, then within the graph by
-// decreasing degree, by (cluster, level), or by id, as its order says (a
-// cluster order whose search stayed shallower than min_levels keeps the ids).
-__global__ __launch_bounds__(256) void k_un_keys(int n, const int* __restrict__ nindex, const int* __restrict__ vgid,
-                                                 const long long* __restrict__ voff, const int* __restrict__ gtype,
-                                                 const int* __restrict__ gdepth, int min_levels,
-                                                 const int* __restrict__ lab, const int* __restrict__ lev,
-                                                 u64* __restrict__ key, int* __restrict__ ids)
-{
-  const int v = blockIdx.x * 256 + threadIdx.x;
-  if (v >= n) return;
-  const int g = vgid[v];
-  int t = gtype[g];
-  if (t == emst::kOrderCluster && gdepth[g] < min_levels) t = 0;
-  u64 local;
-  if (t == emst::kOrderDegree) {
-    local = (u64)(0x7FFFFFFF - (nindex[v + 1] - nindex[v]));
-  } else if (t == emst::kOrderCluster) {
-    local = lev[v] != kUnreached ? ((u64)((lab[v] - voff[g]) / kClusterStride) << 20) | (u64)lev[v]
-                                 : (u64)0xFFFFF << 20;
-  } else {
-    local = (u64)(v - voff[g]);
-  }
-  key[v] = ((u64)g << 40) | local;
-  ids[v] = v;
 }
 
 __global__ __launch_bounds__(256) void k_cl_push(int n, int t, const int* __restrict__ nindex,
@@ -222,14 +181,14 @@ __global__ __launch_bounds__(256) void k_cl_take(int n, int t, int* __restrict__
 }
 
 // Sort keys of the cluster order: (label, level); unreached vertices last.
-__global__ __launch_bounds__(256) void k_cl_keys(int n, const int* __restrict__ lab,
+__global__ __launch_bounds__(256) void k_cl_keys(int n, int nseeds, const int* __restrict__ lab,
                                                  const int* __restrict__ lev, u64* __restrict__ key,
                                                  int* __restrict__ ids)
 {
   const int v = blockIdx.x * 256 + threadIdx.x;
   if (v < n) {
     const bool r = lev[v] != kUnreached;
-    key[v] = ((u64)(r ? lab[v] : n) << 32) | (u64)(unsigned)(r ? lev[v] : 0);
+    key[v] = ((u64)(r ? lab[v] : nseeds) << 32) | (u64)(unsigned)(r ? lev[v] : 0);
     ids[v] = v;
   }
 }
@@ -276,17 +235,14 @@ int relabel_parent(emst_ctx* ctx, const int* parent, int32_t nodes, int32_t* out
   return EMST_OK;
 }
 
-int build_relabel(emst_ctx* ctx, Relabel& rl, int32_t nodes, int32_t edges, const int* d_nindex,
-                  const int* d_nlist, const int* d_eweight, const int* rev, hipStream_t s, const OrderSpec& spec)
+int build_relabel(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
+                  const int* d_eweight, const int* rev, hipStream_t s, int mode, int min_levels)
 {
+  Relabel& rl = ctx->rl;
   rl.ok = false;
   rl.mode = 0;
   rl.levels = 0;
   if (nodes < 2 || edges < 1) return EMST_OK;
-  const int mode = spec.mode;
-  const bool uni = mode == kOrderUnion;
-  if (uni && (!spec.vgid || !spec.voff || !spec.gtype || spec.ngraphs < 1))
-    return ctx_fail(ctx, EMST_E_ARG, "a union order needs its vertex-graph map, offsets and orders");
   const auto t0 = std::chrono::steady_clock::now();
   auto fail = [&](hipError_t e, const char* what) {
     char b[256];
@@ -305,47 +261,42 @@ int build_relabel(emst_ctx* ctx, Relabel& rl, int32_t nodes, int32_t edges, cons
         hipMalloc((void**)&rl.newid, n * 4) || hipMalloc((void**)&rl.nlist, m * 4) ||
         hipMalloc((void**)&rl.eweight, m * 4) || hipMalloc((void**)&rl.kid, m * 4)) {
       free_relabel(rl);
-      return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of the renumbered layout failed");
+      return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of the degree-ordered layout failed");
     }
     rl.cap_nodes = n;
     rl.cap_edges = m;
   }
-  // bits of a vertex id, and of a union's graph index
+  // bits of a vertex id
   int B = 1;
   while (B < 31 && ((size_t)1 << B) < n) B++;
-  int GB = 1;
-  while (uni && GB < 23 && (1 << GB) < spec.ngraphs) GB++;
-  const bool search = mode == kOrderCluster || (uni && spec.any_cluster);
-  const bool vkeys = mode != kOrderDegree;  // the vertex order comes from 64-bit keys
-  const int vkey_bits = uni ? 40 + GB : 32 + B + 1;
-  DevBuf deg, ids, ndeg, k0, k1, v0, v1, tmp, lab, lev, cand, vk0, vk1, cnt, gdepth;
+  const int nseeds = (int)((n + kClusterStride - 1) / kClusterStride);
+  int SB = 1;  // bits of a cluster label (unreached vertices take nseeds)
+  while (SB < 31 && ((size_t)1 << SB) <= (size_t)nseeds) SB++;
+  DevBuf deg, ids, ndeg, k0, k1, v0, v1, tmp, lab, lev, cand, vk0, vk1, cnt;
   if (hipMalloc(&deg.p, n * 4) || hipMalloc(&ids.p, n * 4) || hipMalloc(&ndeg.p, (n + 1) * 4) ||
       hipMalloc(&k0.p, m * 8) || hipMalloc(&k1.p, m * 8) || hipMalloc(&v0.p, m * 4) || hipMalloc(&v1.p, m * 4))
     return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of the relabelling buffers failed");
-  if (vkeys && (hipMalloc(&vk0.p, n * 8) || hipMalloc(&vk1.p, n * 8)))
-    return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of the vertex keys failed");
-  if (search && (hipMalloc(&lab.p, n * 4) || hipMalloc(&lev.p, n * 4) || hipMalloc(&cand.p, n * 4) ||
-                 hipMalloc(&cnt.p, 4) || (uni && hipMalloc(&gdepth.p, (size_t)spec.ngraphs * 4))))
+  if (mode == kOrderCluster && (hipMalloc(&lab.p, n * 4) || hipMalloc(&lev.p, n * 4) || hipMalloc(&cand.p, n * 4) ||
+                                hipMalloc(&vk0.p, n * 8) || hipMalloc(&vk1.p, n * 8) || hipMalloc(&cnt.p, 4)))
     return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of the cluster search buffers failed");
   const int nb = (int)((n + 255) / 256);
   hipLaunchKernelGGL(k_rl_deg, dim3(nb), dim3(256), 0, s, nodes, d_nindex, (int*)deg.p, (int*)ids.p);
   RLCHK(hipGetLastError());
-  size_t b_deg = 0, b_scan = 0, b_key = 0, b_vk = 0;
+  size_t b_deg = 0, b_scan = 0, b_key = 0, b_cl = 0;
   hipcub::DoubleBuffer<u64> dk((u64*)k0.p, (u64*)k1.p);
   hipcub::DoubleBuffer<int> dv((int*)v0.p, (int*)v1.p);
   RLCHK(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, b_deg, (const int*)deg.p, (int*)ndeg.p,
                                                      (const int*)ids.p, rl.order, nodes, 0, 32, s));
   RLCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, b_scan, (const int*)ndeg.p, rl.nindex, nodes + 1, s));
   RLCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b_key, dk, dv, edges, 0, 2 * B, s));
-  if (vkeys)
-    RLCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b_vk, (const u64*)vk0.p, (u64*)vk1.p, (const int*)ids.p,
-                                             rl.order, nodes, 0, vkey_bits, s));
-  if (hipMalloc(&tmp.p, std::max({b_deg, b_scan, b_key, b_vk, (size_t)16})))
+  if (mode == kOrderCluster)
+    RLCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b_cl, (const u64*)vk0.p, (u64*)vk1.p, (const int*)ids.p,
+                                             rl.order, nodes, 0, 32 + SB, s));
+  if (hipMalloc(&tmp.p, std::max({b_deg, b_scan, b_key, b_cl, (size_t)16})))
     return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of the sort storage failed");
-  if (search) {
+  if (mode == kOrderCluster) {
     // breadth-first levels from all seeds at once; one host read per level
-    hipLaunchKernelGGL(k_cl_init, dim3(nb), dim3(256), 0, s, nodes, uni ? spec.vgid : nullptr,
-                       uni ? spec.voff : nullptr, uni ? spec.gtype : nullptr, (int*)lab.p, (int*)lev.p, (int*)cand.p);
+    hipLaunchKernelGGL(k_cl_init, dim3(nb), dim3(256), 0, s, nodes, (int*)lab.p, (int*)lev.p, (int*)cand.p);
     RLCHK(hipGetLastError());
     int t = 1;
     for (;; t++) {
@@ -362,30 +313,13 @@ int build_relabel(emst_ctx* ctx, Relabel& rl, int32_t nodes, int32_t edges, cons
     }
     rl.levels = t - 1;
     // a low-diameter graph gains nothing from clusters: keep the caller's numbering
-    if (!uni && rl.levels < spec.min_levels) return EMST_OK;
-    if (uni) {
-      RLCHK(hipMemsetAsync(gdepth.p, 0, (size_t)spec.ngraphs * 4, s));
-      hipLaunchKernelGGL(k_cl_depth, dim3(nb), dim3(256), 0, s, nodes, spec.vgid, (const int*)lev.p, (int*)gdepth.p);
-      RLCHK(hipGetLastError());
-      rl.gdepth.assign((size_t)spec.ngraphs, 0);
-      RLCHK(hipMemcpyAsync(rl.gdepth.data(), gdepth.p, (size_t)spec.ngraphs * 4, hipMemcpyDeviceToHost, s));
-      RLCHK(hipStreamSynchronize(s));
-    }
-  }
-  if (uni) {
-    hipLaunchKernelGGL(k_un_keys, dim3(nb), dim3(256), 0, s, nodes, d_nindex, spec.vgid, spec.voff, spec.gtype,
-                       (const int*)gdepth.p, spec.min_levels, (const int*)lab.p, (const int*)lev.p, (u64*)vk0.p,
-                       (int*)ids.p);
-    RLCHK(hipGetLastError());
-  } else if (mode == kOrderCluster) {
-    hipLaunchKernelGGL(k_cl_keys, dim3(nb), dim3(256), 0, s, nodes, (const int*)lab.p, (const int*)lev.p,
+    if (rl.levels < min_levels) return EMST_OK;
+    hipLaunchKernelGGL(k_cl_keys, dim3(nb), dim3(256), 0, s, nodes, nseeds, (const int*)lab.p, (const int*)lev.p,
                        (u64*)vk0.p, (int*)ids.p);
     RLCHK(hipGetLastError());
-  }
-  if (vkeys) {
-    // stable: ties keep the caller's order
-    RLCHK(hipcub::DeviceRadixSort::SortPairs(tmp.p, b_vk, (const u64*)vk0.p, (u64*)vk1.p, (const int*)ids.p,
-                                             rl.order, nodes, 0, vkey_bits, s));
+    // stable: within a level of a cluster, the caller's order
+    RLCHK(hipcub::DeviceRadixSort::SortPairs(tmp.p, b_cl, (const u64*)vk0.p, (u64*)vk1.p, (const int*)ids.p,
+                                             rl.order, nodes, 0, 32 + SB, s));
     hipLaunchKernelGGL(k_rl_ndeg, dim3(nb), dim3(256), 0, s, nodes, rl.order, d_nindex, (int*)ndeg.p);
     RLCHK(hipGetLastError());
   } else {

@@ -166,8 +166,7 @@ _bf16_bits = _sig("emst_bf16_bits", C.c_uint16, C.c_float)
 class BatchStats(C.Structure):
     _fields_ = [("ms", C.c_double), ("chunks", C.c_int32), ("graphs", C.c_int32), ("nodes", C.c_int64),
                 ("edges", C.c_int64), ("light", C.c_int64), ("heavy", C.c_int64), ("light_rounds", C.c_int32),
-                ("heavy_rounds", C.c_int32), ("replays", C.c_int32), ("filtered_graphs", C.c_int32),
-                ("renumbered_graphs", C.c_int32)]
+                ("heavy_rounds", C.c_int32), ("replays", C.c_int32), ("filtered_graphs", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -141,38 +141,3 @@ def test_batch_learned_graph_replay(setup):
     assert st.replays == st.chunks
     check_all(b, graphs, refs)
     b.close()
-
-
-def test_batch_renumbered_arenas():
-    """A batch arena is renumbered graph by graph, each graph in its own
-    vertex range, as emst_load would renumber it alone: an RMAT graph by
-    decreasing degree, a geometric graph of 2^17 vertices in cluster order;
-    an Erdos-Renyi graph of that size (shallow search), a grid and small
-    graphs keep their ids.  Every graph's inMST is Kruskal's, in every mode,
-    with and without hipGraph replay, and the same with EMST_F_NO_RELABEL."""
-    s = emst.Solver(0)
-    s.set_filter_model(np.fromfile(MODEL, np.float32))
-    graphs = [emst.gen_rmat(12, 8, 0.57, 0.19, 0.19, 3, 1, 1 << 20), emst.gen_rgg(1 << 17, 6.0, 7, 1 << 20),
-              emst.gen_er(1 << 17, 10.0, 8, 1, 1 << 20), emst.gen_grid(60, 50, 9, 1, 100),
-              emst.gen_rmat(13, 8, 0.57, 0.19, 0.19, 4, 1, 9)] + mixed_graphs(12, 11)
-    refs = [O.kruskal(g) for g in graphs]
-    try:
-        for flags, want in ((0, None), (emst.F_NO_RELABEL, 0)):
-            s.set_config(flags=flags)
-            print("flags", flags, flush=True)
-            b = emst.Batch(s, graphs, max_arena_edges=1 << 22)
-            s.set_config()
-            for mode in (emst.FILTER_OFF, emst.FILTER_PLAIN, emst.FILTER_LEARNED):
-                for use_graph in (False, True, True):
-                    st = b.solve(mode, use_graph=use_graph)
-                    check_all(b, graphs, refs)
-                    if want is None:
-                        # the two RMAT graphs, the geometric graph, and the
-                        # mixed RMAT graphs of at least 1024 vertices
-                        assert st.renumbered_graphs >= 3, st.as_dict()
-                    else:
-                        assert st.renumbered_graphs == 0
-            b.close()
-    finally:
-        s.set_config()
-        s.close()
