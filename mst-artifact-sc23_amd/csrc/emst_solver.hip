@@ -2112,7 +2112,15 @@ int emst_load(emst_ctx* ctx, const emst_graph* g)
   if (order) {
     const int rc3 = build_relabel(ctx, g->nodes, g->edges, ctx->d_nindex, ctx->d_nlist, ctx->d_eweight, ctx->d_rev,
                                   ctx->stream, order, min_levels);
-    if (rc3) return rc3;
+    if (rc3 == EMST_E_NOMEM) {
+      // the copy is an optimisation: without room for it, solve in the
+      // caller's numbering
+      free_relabel(ctx->rl);
+      ctx->err.clear();
+      (void)hipGetLastError();  // the failed allocation's error is not the solve's
+    } else if (rc3) {
+      return rc3;
+    }
   }
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->n = g->nodes;
