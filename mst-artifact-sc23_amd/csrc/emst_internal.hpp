@@ -62,8 +62,9 @@ struct emst_ctx {
   size_t cap_rev = 0;
   bool rev_ok = false;
   bool rev_mirrored = false;
-  // degree-ordered layout of the loaded graph (emst_load, skewed degrees):
-  // vertices renumbered by decreasing degree, lists sorted by the new ids,
+  // renumbered layout of the loaded graph (emst_load: degree order for
+  // skewed degrees, cluster order for scattered ids on a deep graph), lists
+  // sorted by the new ids,
   // each position's key id (its edge's upper-direction position in the
   // caller's CSR) and that copy's weight; order maps new ids to the
   // caller's, newid the reverse
@@ -86,7 +87,7 @@ struct emst_ctx {
   hipEvent_t ev[8] = {};
   int32_t last_nodes = 0;
   const int* last_parent = nullptr;   // parent array of the last solve (context or scratch)
-  bool last_relabelled = false;       // ... in the degree-ordered numbering
+  bool last_relabelled = false;       // ... in the renumbered layout's numbering
   // learned threshold
   unsigned short* d_mlp16 = nullptr;  // bf16 parameters
   int have_model = 0;
@@ -200,7 +201,7 @@ constexpr int kOrderDegree = 1, kOrderCluster = 2;
 int build_relabel(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nindex, const int* d_nlist,
                   const int* d_eweight, const int* rev, hipStream_t s, int mode, int min_levels);
 void free_relabel(Relabel& rl);
-// Copies a parent array of the degree-ordered layout to host memory in the
+// Copies a parent array of the renumbered layout to host memory in the
 // caller's numbering.
 int relabel_parent(emst_ctx* ctx, const int* parent, int32_t nodes, int32_t* out, hipStream_t s);
 

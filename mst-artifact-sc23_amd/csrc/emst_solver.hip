@@ -2135,7 +2135,7 @@ int emst_solve(emst_ctx* ctx, int use_filter, int32_t threshold, emst_stats* sta
   if (!ctx) return EMST_E_ARG;
   if (!ctx->loaded) return ctx_fail(ctx, EMST_E_STATE, "emst_solve before emst_load");
   HIPCHK(hipSetDevice(ctx->device));
-  // the degree-ordered copy, unless the ladder (which keys by position) or
+  // the renumbered copy, unless the ladder (which keys by position) or
   // the caller's numbering is asked for
   const Relabel& rl = ctx->rl;
   const bool use_rl = rl.ok && !(ctx->cfg.flags & (kLadderFlags | EMST_F_NO_RELABEL));

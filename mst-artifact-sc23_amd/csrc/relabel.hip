@@ -1,16 +1,22 @@
-// relabel.hip -- the degree-ordered layout of a loaded graph.
+// relabel.hip -- renumbered layouts of a loaded graph.
 //
 // The solver's vertex-indexed arrays (parent, minv, the class map, the
-// label and member flags) are gathered at random by every round.  On a graph
-// with skewed degrees (RMAT) most records touch a few hundred thousand hub
-// vertices, but the caller's numbering scatters them over the whole id
-// range, so each gather fetches a line that holds one hot entry.  emst_load
-// therefore renumbers such a graph by decreasing degree (ties keep the
-// caller's order) and rebuilds its CSR in the new numbering with every list
-// sorted by the new ids.  The hubs' entries then share lines and stay in
-// L2; the hubs' lists also come first, so the largest light-phase component
-// (the hubs) owns a contiguous run of positions that the sparse heavy
-// initialisation skips whole.
+// label and member flags) are gathered at every record's endpoints in every
+// round, so the vertex numbering decides how many lines those gathers touch.
+// emst_load therefore rebuilds the CSR of some graphs in a new numbering,
+// with every list sorted by the new ids, and solves on that copy:
+//  - degree order, for skewed degrees (RMAT).  Most records touch a few
+//    hundred thousand hubs, but the caller's numbering scatters them over
+//    the whole id range, so each gather fetches a line that holds one hot
+//    entry.  Renumbered by decreasing degree (ties keep the caller's order),
+//    the hubs' entries share lines and stay in L2; their lists also come
+//    first, so the largest light-phase component (the hubs) owns a
+//    contiguous run of positions that the sparse heavy initialisation skips
+//    whole.
+//  - cluster order, for scattered ids on a graph of high diameter (a
+//    geometric graph numbered at random).  A breadth-first search from every
+//    4096th vertex at once grows clusters of nearby vertices; numbered by
+//    (cluster, level), most edges join ids of one cluster.
 //
 // Results are unchanged.  The reference keys an edge by its weight and the
 // position j of its upper-direction copy (ECL_MST.cu:173-176, 222-223) and
@@ -23,13 +29,14 @@
 // offers read the key ids with their task; the plain streaming passes emit
 // positions, and the phase's first select replaces each record's position by
 // its key id before any key is formed (a gather there overlaps the select's
-// own gathers; in the streaming pass it stalled the stream).  Both copies of an edge carry
-// its upper copy's weight, the one the reference reads.  Root ids are the
-// new numbering's: emst_fetch_parent maps them back.
+// own gathers; in the streaming pass it stalled the stream).  Both copies of
+// an edge carry its upper copy's weight, the one the reference reads.  Root
+// ids are the new numbering's: emst_fetch_parent maps them back.
 //
-// Built on the device from the uploaded CSR and its reverse positions: one
-// 64-bit radix sort of (new source, new neighbour) keys with the caller's
-// positions as values.
+// Built on the device from the uploaded CSR and its reverse positions: the
+// vertex order (a radix sort of degrees, or the cluster search and a radix
+// sort of its keys), then one 64-bit radix sort of (new source, new
+// neighbour) keys with the caller's positions as values.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -223,7 +230,7 @@ void free_relabel(Relabel& rl)
 int relabel_parent(emst_ctx* ctx, const int* parent, int32_t nodes, int32_t* out, hipStream_t s)
 {
   const Relabel& rl = ctx->rl;
-  if (!rl.ok || (size_t)nodes > rl.cap_nodes) return ctx_fail(ctx, EMST_E_STATE, "no degree-ordered layout");
+  if (!rl.ok || (size_t)nodes > rl.cap_nodes) return ctx_fail(ctx, EMST_E_STATE, "no renumbered layout");
   DevBuf tmp;
   if (hipMalloc(&tmp.p, (size_t)std::max(nodes, 1) * 4)) return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc failed");
   hipLaunchKernelGGL(k_rl_parent, dim3((nodes + 255) / 256), dim3(256), 0, s, nodes, parent, rl.newid, rl.order,
@@ -261,7 +268,7 @@ int build_relabel(emst_ctx* ctx, int32_t nodes, int32_t edges, const int* d_nind
         hipMalloc((void**)&rl.newid, n * 4) || hipMalloc((void**)&rl.nlist, m * 4) ||
         hipMalloc((void**)&rl.eweight, m * 4) || hipMalloc((void**)&rl.kid, m * 4)) {
       free_relabel(rl);
-      return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of the degree-ordered layout failed");
+      return ctx_fail(ctx, EMST_E_NOMEM, "hipMalloc of the renumbered layout failed");
     }
     rl.cap_nodes = n;
     rl.cap_edges = m;
