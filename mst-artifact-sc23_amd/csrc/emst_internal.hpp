@@ -63,11 +63,10 @@ struct emst_ctx {
   bool rev_ok = false;
   bool rev_mirrored = false;
   // renumbered layout of the loaded graph (emst_load: degree order for
-  // skewed degrees, cluster order for scattered ids on a deep graph), lists
-  // sorted by the new ids,
-  // each position's key id (its edge's upper-direction position in the
-  // caller's CSR) and that copy's weight; order maps new ids to the
-  // caller's, newid the reverse
+  // skewed degrees, cluster order for scattered ids on a deep graph): lists
+  // sorted by the new ids, each position's key id (its edge's
+  // upper-direction position in the caller's CSR) and that copy's weight;
+  // order maps new ids to the caller's, newid the reverse
   Relabel rl;
   int* d_cnt = nullptr;
   long long* d_stats = nullptr;
