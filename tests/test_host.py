@@ -1,6 +1,7 @@
 """Host-side tests of libemst.so (no GPU): exported C-ABI, .egr reader and
 writer, weight synthesis, plain threshold, host Kruskal, validation, the
 workload generators, and the CLI's pre-device error paths."""
+import ctypes as C
 import hashlib
 import json
 import os
@@ -520,3 +521,21 @@ def test_threshold_cost_model_calibration_on_synthetic_times():
     assert y[1] == np.float32(TF.OFF_LABEL) and y[0] == -1.0 and y[2] == -2.0
     assert abs((y[0] + y[1]) / 2 - np.log2(0.9)) < 0.01
     assert TF.label_of(np.array([1.0]), "union")[0] == 0.0
+
+
+def test_binding_structs_match_header_layouts():
+    """The ctypes mirrors of emst_stats, emst_batch_stats and emst_config
+    list the header's fields in order, with C sizes that match (a field
+    added on one side only would shift every field after it)."""
+    text = open(os.path.join(ROOT, "include", "emst.h")).read()
+    ctype = {"double": C.c_double, "int32_t": C.c_int32, "int64_t": C.c_int64, "uint64_t": C.c_uint64,
+             "float": C.c_float}
+    for struct, mirror in (("emst_stats", emst.Stats), ("emst_batch_stats", emst.BatchStats),
+                           ("emst_config", emst.Config)):
+        body = re.search(r"typedef struct \{([^}]*)\} " + struct + ";", text).group(1)
+        body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+        fields = [(t, n.strip()) for t, names in re.findall(r"^\s*(\w+)\s+([\w\s,]+);", body, flags=re.M)
+                  for n in names.split(",")]
+        assert [n for _, n in fields] == [n for n, _ in mirror._fields_], struct
+        for (t, n), (_, ct) in zip(fields, mirror._fields_):
+            assert C.sizeof(ctype[t]) == C.sizeof(ct), (struct, n)
