@@ -9,7 +9,10 @@ ECL_MST.cu:301-345).  Two paths are measured, both exact:
              network (mst-artifact-sc23_amd/filter_mlp.f32); the headline
   plain   -- the reference's threshold rule (ECL_MST.cu:432-443)
 The metric is directed CSR edges per second (generate_compare_tables.py:
-55-59 reports Medges/s; this reports Gedges/s).
+55-59 reports Medges/s; this reports Gedges/s).  emst_load renumbers the
+RMAT graphs by degree and the geometric graph by clusters, on the device and
+outside the timed region (its cost is in pcie_inclusive's load_ms); the
+line's `layout` record times the same solve in the caller's numbering.
 
 The default run also carries two sub-records, the rest of the headline
 metric: "rgg24" (the random geometric graph of 2^24 vertices, learned
