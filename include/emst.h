@@ -220,7 +220,11 @@ int emst_gpu_mst(emst_ctx* ctx, const emst_graph* g, int use_filter, int32_t thr
  * once (the network's inference on the device), before its timed region,
  * and reuses it until the next emst_load or emst_set_filter_model, as the
  * reference decides its threshold before calling gpuMST (ECL_MST.cu:432-441);
- * emst_solve_device decides it inside every solve. */
+ * emst_solve_device decides it inside every solve.  For sorted, symmetric
+ * lists emst_load also builds on the device the reverse positions (the
+ * sparse heavy pass and the fused light pass use them) and, by the rules of
+ * EMST_F_NO_RELABEL, a renumbered copy of the CSR that the solves then run
+ * on; results are in the caller's layout either way. */
 int emst_load(emst_ctx* ctx, const emst_graph* g);
 int emst_solve(emst_ctx* ctx, int use_filter, int32_t threshold, emst_stats* stats);
 int emst_fetch(emst_ctx* ctx, uint8_t* inMST);
