@@ -287,8 +287,13 @@ static int run_rounds(orc_rec* wl, size_t len, int32_t* parent, uint64_t* minv,
   return 0;
 }
 
-int orc_ecl_schedule(const orc_graph* g, int use_filter, int32_t threshold,
-                     uint8_t* inMST, orc_sched* st)
+/* The schedule with every position keyed by an id of the caller's choosing:
+ * kid == NULL keys position j by j (the reference, :173-176); otherwise by
+ * kid[j], the key id a renumbered copy of a graph carries (the position of
+ * the edge's upper-direction copy in the original CSR), and inMST is indexed
+ * by those ids. */
+static int schedule_impl(const orc_graph* g, int use_filter, int32_t threshold, const int32_t* kid,
+                         uint8_t* inMST, orc_sched* st)
 {
   const int32_t n = g->nodes, m = g->edges;
   memset(st, 0, sizeof(*st));
@@ -310,7 +315,7 @@ int orc_ecl_schedule(const orc_graph* g, int use_filter, int32_t threshold,
     for (int32_t j = g->nindex[v]; j < g->nindex[v + 1]; j++) {
       const int32_t u = g->nlist[j];
       if (u > v && g->eweight[j] <= T) {
-        wl[len].x = v; wl[len].y = u; wl[len].key = ecl_key(g->eweight[j], j); len++;
+        wl[len].x = v; wl[len].y = u; wl[len].key = ecl_key(g->eweight[j], kid ? kid[j] : j); len++;
       }
     }
   st->light = (int64_t)len;
@@ -327,7 +332,9 @@ int orc_ecl_schedule(const orc_graph* g, int use_filter, int32_t threshold,
         if (u > v && g->eweight[j] > T) {
           st->heavy_all++;
           const int32_t b = uf_find(u, parent);
-          if (a != b) { wl[len].x = a; wl[len].y = b; wl[len].key = ecl_key(g->eweight[j], j); len++; }
+          if (a != b) {
+            wl[len].x = a; wl[len].y = b; wl[len].key = ecl_key(g->eweight[j], kid ? kid[j] : j); len++;
+          }
         }
       }
     }
@@ -338,12 +345,27 @@ int orc_ecl_schedule(const orc_graph* g, int use_filter, int32_t threshold,
   int64_t comps = 0, mst = 0;
   uint64_t wsum = 0;
   for (int32_t v = 0; v < n; v++) if (uf_find(v, parent) == v) comps++;
-  for (int32_t j = 0; j < m; j++) if (inMST[j]) { mst++; wsum += (uint64_t)(int64_t)g->eweight[j]; }
+  /* marked edges and their weights, counted at each edge's upper copy */
+  for (int32_t v = 0; v < n; v++)
+    for (int32_t j = g->nindex[v]; j < g->nindex[v + 1]; j++)
+      if (g->nlist[j] > v && inMST[kid ? kid[j] : j]) { mst++; wsum += (uint64_t)(int64_t)g->eweight[j]; }
   st->components = comps;
   st->mst_edges = mst;
   st->weight = wsum;
   free(parent); free(minv); free(wl);
   return 0;
+}
+
+int orc_ecl_schedule(const orc_graph* g, int use_filter, int32_t threshold,
+                     uint8_t* inMST, orc_sched* st)
+{
+  return schedule_impl(g, use_filter, threshold, NULL, inMST, st);
+}
+
+int orc_ecl_schedule_keyed(const orc_graph* g, int use_filter, int32_t threshold, const int32_t* kid,
+                           uint8_t* inMST, orc_sched* st)
+{
+  return schedule_impl(g, use_filter, threshold, kid, inMST, st);
 }
 
 /* Restates verify (ECL_MST.cu:365-386).  Returns 0 when both edge sets and

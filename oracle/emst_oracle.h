@@ -54,6 +54,10 @@ void orc_plain_threshold(int32_t nodes, int32_t edges, const int32_t* eweight,
 int orc_kruskal(const orc_graph* g, uint8_t* inMST);
 int orc_ecl_schedule(const orc_graph* g, int use_filter, int32_t threshold,
                      uint8_t* inMST, orc_sched* st);
+/* The same schedule with position j keyed by kid[j] and inMST indexed by
+ * those ids (a renumbered copy of a graph keyed by its original positions). */
+int orc_ecl_schedule_keyed(const orc_graph* g, int use_filter, int32_t threshold, const int32_t* kid,
+                           uint8_t* inMST, orc_sched* st);
 int orc_verify(const orc_graph* g, const uint8_t* cpu, const uint8_t* gpu,
                uint64_t* cpu_weight, uint64_t* gpu_weight, int64_t* only_cpu, int64_t* only_gpu);
 

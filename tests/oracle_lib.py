@@ -49,6 +49,7 @@ lib.orc_hash.restype = C.c_uint32
 lib.orc_plain_threshold.argtypes = [C.c_int32, C.c_int32, _p32, C.POINTER(C.c_int), C.POINTER(C.c_int32)]
 lib.orc_kruskal.argtypes = [C.POINTER(_G), _pu8]
 lib.orc_ecl_schedule.argtypes = [C.POINTER(_G), C.c_int, C.c_int32, _pu8, C.POINTER(Sched)]
+lib.orc_ecl_schedule_keyed.argtypes = [C.POINTER(_G), C.c_int, C.c_int32, _p32, _pu8, C.POINTER(Sched)]
 lib.orc_verify.argtypes = [C.POINTER(_G), _pu8, _pu8, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                            C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
 
@@ -117,6 +118,19 @@ def schedule(g, use_filter, threshold):
     st = Sched()
     rc = lib.orc_ecl_schedule(C.byref(_view(ni, nl, ew)), int(use_filter), int(threshold),
                               out.ctypes.data_as(_pu8), C.byref(st))
+    assert rc == 0
+    return out[:len(nl)], st
+
+
+def schedule_keyed(g, use_filter, threshold, kid):
+    """orc_ecl_schedule with position j keyed by kid[j]; inMST is indexed
+    by the key ids."""
+    ni, nl, ew = _arrs(g)
+    k = np.ascontiguousarray(kid, np.int32)
+    out = np.zeros(max(len(nl), 1), np.uint8)
+    st = Sched()
+    rc = lib.orc_ecl_schedule_keyed(C.byref(_view(ni, nl, ew)), int(use_filter), int(threshold),
+                                    k.ctypes.data_as(_p32), out.ctypes.data_as(_pu8), C.byref(st))
     assert rc == 0
     return out[:len(nl)], st
 

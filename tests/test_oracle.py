@@ -286,3 +286,87 @@ def test_forest_check_pins():
         if len(extra):                       # add a non-tree edge: a cycle
             bad = k.copy(); bad[extra[0]] = 1
             assert not O.forest_check(g, bad)[2]
+
+
+def _renumber(g, order):
+    """A renumbered copy of a sorted, symmetric CSR (numpy restatement of
+    the solver's load-time copy): new id i is the caller's order[i]; every
+    list is sorted by new neighbour id (ties keep the caller's positions);
+    each position carries its edge's key id kid (the caller's position of
+    the upper-direction copy; the k-th copy of y in x's list pairs with the
+    k-th copy of x in y's list) and that copy's weight."""
+    n, m = len(g.nindex) - 1, len(g.nlist)
+    deg = np.diff(g.nindex)
+    src = np.repeat(np.arange(n), deg)
+    nl = g.nlist.astype(np.int64)
+    pos = np.arange(m)
+    lo, hi = np.minimum(src, nl), np.maximum(src, nl)
+    up = nl > src
+    kid = pos.copy()
+    # rank of each copy among the copies of its pair, in position order
+    pair = lo * n + hi
+    o = np.lexsort((pos, up, pair))
+    ps, us = pair[o], up[o]
+    grp_start = np.r_[0, np.flatnonzero((ps[1:] != ps[:-1]) | (us[1:] != us[:-1])) + 1]
+    rank = np.empty(m, np.int64)
+    rank[o] = np.arange(m) - np.repeat(grp_start, np.diff(np.r_[grp_start, m]))
+    upper_of = {}
+    for p in np.flatnonzero(up):
+        upper_of[(int(pair[p]), int(rank[p]))] = int(p)
+    for p in np.flatnonzero((~up) & (nl != src)):
+        kid[p] = upper_of[(int(pair[p]), int(rank[p]))]
+    newid = np.empty(n, np.int64)
+    newid[order] = np.arange(n)
+    ns, nd = newid[src], newid[nl]
+    o = np.lexsort((pos, nd, ns))
+    ni = np.zeros(n + 1, np.int64)
+    np.add.at(ni, ns + 1, 1)
+    ni = np.cumsum(ni)
+    w = g.eweight[kid[o]]
+    return G(ni.astype(np.int32), nd[o].astype(np.int32), w.astype(np.int32)), kid[o].astype(np.int32)
+
+
+def test_schedule_is_invariant_under_renumbering_with_key_ids():
+    """The mathematics behind the solver's renumbered copies: keyed by
+    (weight, position of the upper-direction copy) the keys are distinct, so
+    every phase's forest is the unique minimum forest under that order, and
+    the round structure (records per round, L, H, rounds) depends only on
+    which components merge, not on vertex ids.  The schedule run on a
+    renumbered copy whose positions carry the original key ids (and the
+    upper copies' weights) must therefore mark exactly the original inMST
+    bytes with the same statistics: random permutations and the degree order,
+    on multigraphs with ties, self-loops and copies whose weights differ."""
+    rng = np.random.default_rng(53)
+    for it in range(40):
+        n = int(rng.integers(2, 120))
+        ni, nl, ew = random_graph(rng, n, int(rng.integers(0, 400)), 1, 6, allow_loops=True)
+        ew = ew.copy()
+        src = np.repeat(np.arange(n), np.diff(ni))
+        if it % 2:
+            low = nl < src  # lower copies disagree: only the upper copy's weight counts
+            ew[low] = rng.integers(1, 6, int(low.sum()))
+        g = G(ni, nl, ew)
+        w = np.sort(g.eweight) if len(g.nlist) else np.zeros(1, np.int32)
+        deg = np.diff(ni)
+        for order in (np.argsort(-deg, kind="stable"), rng.permutation(n)):
+            g2, kid = _renumber(g, order)
+            for uf, T in ((False, 2 ** 31 - 1), (True, int(w[len(w) // 3])), (True, int(w[-1]))):
+                ref, st = O.schedule(g, uf, T)
+                got, st2 = O.schedule_keyed(g2, uf, T, kid)
+                assert np.array_equal(got, ref), (it, uf, T)
+                assert (st.light, st.heavy, st.light_rounds, st.heavy_rounds, st.components, st.mst_edges,
+                        st.weight) == (st2.light, st2.heavy, st2.light_rounds, st2.heavy_rounds, st2.components,
+                                       st2.mst_edges, st2.weight)
+                assert st.rounds(0) == st2.rounds(0) and st.rounds(1) == st2.rounds(1)
+    # a permutation that keys by the new positions instead (no key ids) is
+    # not invariant on such ties: the check above is not vacuous
+    g = G(*random_graph(np.random.default_rng(59), 60, 300, 1, 3))
+    g2, kid = _renumber(g, np.random.default_rng(61).permutation(60))
+    diff = 0
+    for uf, T in ((False, 2 ** 31 - 1),):
+        ref, _ = O.schedule(g, uf, T)
+        got_pos, _ = O.schedule(g2, uf, T)
+        mapped = np.zeros_like(ref)
+        mapped[kid[got_pos.astype(bool)]] = 1
+        diff += int((mapped != ref).sum())
+    assert diff > 0
