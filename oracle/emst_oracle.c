@@ -27,6 +27,12 @@
  *                        non-negative (the MSF under the (w, j) order is
  *                        unique); per-round sizes checked against a hand
  *                        trace of a path graph and against invariants.
+ *   orc_ecl_schedule_keyed: with kid[j] = j it is orc_ecl_schedule; on a
+ *                        renumbered copy keyed by the original positions it
+ *                        reproduces the original bytes and round sizes (the
+ *                        keys are distinct, so each phase's forest and round
+ *                        structure are fixed), while keying the copy by its
+ *                        own positions does not.
  *   orc_verify         : direct cases (equal, one flipped bit, weight sum).
  *   orc_forest_check   : Kruskal's forest passes; dropping an edge breaks
  *                        spanning, adding a non-tree edge makes a cycle.
